@@ -1,0 +1,151 @@
+/* pgx.h — C ABI of the MI355X (gfx950) batched probable-prime tester.
+ *
+ * Plain C types only (pointers, sizes, status codes).  Every entry point
+ * returns PGX_OK (0) or a negative PGX_E_* code; nothing calls exit() and no
+ * C++ exception crosses this boundary.  The per-runner message behind a failing
+ * call is available from pgx_last_error().
+ *
+ * The entry points replace the CUDA runner of the reference, one for one:
+ *
+ *   reference (sethtroisi/prime-gap-gpu)                     here
+ *   ------------------------------------------------------   ----------------------------
+ *   test_runner_t<params>::test_runner_t(n, rounds)          pgx_runner_create
+ *     miller_rabin.h:322-349 (pinned + device buffers,
+ *     base list generate_primes(rounds) :272-289, stream)
+ *   test_runner_t::~test_runner_t   miller_rabin.h:351-362   pgx_runner_destroy
+ *   test_runner_t::run_test         miller_rabin.h:364-408   pgx_submit + pgx_wait, or pgx_run
+ *     (pack mpz -> BITS/32 LE u32 limbs :373-378, H2D :383,
+ *      kernel_miller_rabin :389-394, D2H :398, sync :402,
+ *      results[i] = (passed == rounds) :405-407)
+ *   CUDA_CHECK / CGBN_CHECK / from_mpz exit(1)               negative status + pgx_last_error
+ *     miller_rabin.h:43-97
+ *   kernel_miller_rabin<params>     miller_rabin.h:252-270   pgx_test_device (device-resident
+ *                                                            inputs, caller's stream)
+ *   fill_batch (N = m*K + x)        gpu_testing.cu:136-195   pgx_gen_* (host candidate builder)
+ *
+ * Limb layout.  Host arrays passed to pgx_submit/pgx_run are candidate-major,
+ * exactly the reference's instance_t limbs: candidate i, limb j (little-endian
+ * 32-bit words, mpz_export(order=-1, size=4)) at [i * limbs + j].  Device
+ * arrays (pgx_test_device) are limb-interleaved: limb j of candidate i at
+ * [j * ld + i], so a wavefront reads each limb row with one coalesced load.
+ */
+#ifndef PGX_H
+#define PGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PGX_OK 0
+#define PGX_E_ARG (-1)      /* bad argument (null pointer, zero size, ...) */
+#define PGX_E_HIP (-2)      /* HIP runtime failure (message names the call) */
+#define PGX_E_CAPACITY (-3) /* n larger than the runner's max_n (miller_rabin.h:368-371) */
+#define PGX_E_INPUT (-4)    /* candidate even, <= 1, or too wide (miller_rabin.h:46-49, 375-376) */
+#define PGX_E_STATE (-5)    /* wait without submit, submit while busy */
+#define PGX_E_NODEV (-6)    /* no usable gfx950 device */
+
+/* test modes (flags & PGX_MODE_MASK) */
+#define PGX_MODE_REFERENCE 0u /* reference verdict: passed = number of leading bases that pass
+                                 the strong test as written in miller_rabin.h:216-248 (which
+                                 also accepts a^(N-1) == -1 when ctz(N-1) >= 2) */
+#define PGX_MODE_STRONG 1u    /* textbook strong test per base: passed = leading bases passed */
+#define PGX_MODE_FERMAT 2u    /* base-2 Fermat only: verdict word = PGX_FLAG_* bits of base 2 */
+#define PGX_MODE_MASK 3u
+#define PGX_WANT_RESIDUE 4u   /* also return a^(N-1) mod N of the first base (usually 2) */
+
+/* bits of the verdict word in PGX_MODE_FERMAT */
+#define PGX_FLAG_FERMAT 1u /* 2^(N-1) == 1 (mod N) */
+#define PGX_FLAG_SPRP 2u   /* strong probable prime to base 2 */
+#define PGX_FLAG_QUIRK 4u  /* reference verdict for base 2 */
+
+#define PGX_VERDICT_INVALID 0xFFFFFFFFu /* candidate rejected by the kernel */
+#define PGX_MAX_BASES 16
+#define PGX_MAX_KERNEL_LIMBS 64 /* N < 2^(32*64 - 2) */
+
+typedef struct pgx_runner pgx_runner_t;
+
+/* Create a runner on `device` for up to max_n candidates stored in `limbs`
+ * 32-bit words each (the reference's BITS/32).  `bases` lists the Miller-Rabin
+ * bases in test order (the reference passes generate_primes(rounds) = the
+ * first `rounds` primes, miller_rabin.h:272-289); NULL with nbases = r means
+ * the first r primes, as there.  Replaces test_runner_t's constructor,
+ * miller_rabin.h:322-349. */
+int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t limbs,
+                      const uint32_t* bases, uint32_t nbases);
+
+/* Free everything the runner owns.  Replaces miller_rabin.h:351-362. */
+int pgx_runner_destroy(pgx_runner_t* h);
+
+/* Pack n candidates (candidate-major limbs, see above) into pinned staging,
+ * copy them to the device and launch asynchronously on the runner's stream.
+ * Rejects even N, N <= 1 and N >= 2^(32*limbs) with PGX_E_INPUT (the reference
+ * asserts / exits, miller_rabin.h:46-49, 375-376). */
+int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t flags);
+
+/* Wait for the submitted batch.  verdict[i] (n entries): the mode's verdict
+ * word (reference mode: leading bases passed, compare with nbases as
+ * miller_rabin.h:406 does).  residues (n * limbs words, candidate-major) is
+ * filled when PGX_WANT_RESIDUE was submitted, else may be NULL. */
+int pgx_wait(pgx_runner_t* h, uint32_t* verdict, uint32_t* residues);
+
+/* pgx_submit + pgx_wait. */
+int pgx_run(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t flags,
+            uint32_t* verdict, uint32_t* residues);
+
+/* Device-resident batch: candidates already in HBM, limb-interleaved with
+ * leading dimension ld >= n and `limbs` rows; max_bits bounds the bit length of
+ * every candidate (it selects the kernel's limb count S = ceil((max_bits+2)/32)).
+ * verdict: n words; residues: `limbs` rows of res_ld words or NULL.  Launches
+ * on `stream` (a hipStream_t; NULL = the runner's stream) and returns without
+ * waiting.  Replaces kernel_miller_rabin's launch, miller_rabin.h:388-394. */
+int pgx_test_device(pgx_runner_t* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs,
+                    size_t n, uint32_t max_bits, uint32_t flags, uint32_t* d_verdict,
+                    uint32_t* d_residues, uint32_t res_ld, void* stream);
+
+/* Message of the last failing call on h (or of the last failing create when
+ * h is NULL).  Never NULL. */
+const char* pgx_last_error(const pgx_runner_t* h);
+
+/* ---- candidate construction: N = m * K + X with K = P# / d ----------------
+ * Host-side builder for synthetic sieved candidates, the role of fill_batch
+ * (gpu_testing.cu:136-195) and init_K (gap_common.cpp:60-65).
+ *
+ * pgx_gen_sieved: for each of the n_m multipliers m[k] (already chosen by the
+ * caller), sieves X in [1, x_max] with every prime below sieve_limit and keeps
+ * the first per_m survivors in ascending X (N has no prime factor <
+ * sieve_limit).  Writes them in m order: out_limbs (candidate-major, `limbs`
+ * words each), out_m (index k into m[]), out_x.  Returns the number written
+ * (<= n_m * per_m) through *written.  Fails with PGX_E_INPUT if d does not
+ * divide P# or a candidate does not fit in `limbs` words. */
+int pgx_gen_sieved(uint32_t P, uint32_t d, const uint64_t* m, uint32_t n_m, uint32_t x_max,
+                   uint32_t per_m, uint32_t sieve_limit, uint32_t limbs, uint32_t* out_limbs,
+                   uint32_t* out_m, uint32_t* out_x, size_t* written);
+
+/* Draw n_m distinct multipliers uniformly from [m_lo, m_hi) coprime to d with
+ * the generator's seeded xoshiro256** stream (rejection sampling, ascending
+ * order on return). */
+int pgx_gen_multipliers(uint64_t seed, uint64_t m_lo, uint64_t m_hi, uint32_t d, uint32_t n_m,
+                        uint64_t* out_m);
+
+/* Bit length of K = P# / d (0 if d does not divide P#). */
+uint32_t pgx_k_bits(uint32_t P, uint32_t d);
+
+/* ---- measurement --------------------------------------------------------
+ * VALU issue-rate microbenchmark (no counterpart in the reference, which
+ * publishes no roofline): lane-instructions per second of one instruction
+ * kind on `device`, 8 independent chains per lane, `blocks` x 256 threads x
+ * `iters` x 8 instructions.  op: 0 v_mad_u64_u32, 1 v_mad_u64_u32 +
+ * v_addc_co_u32 (one multiply-accumulate of the kernels), 2 v_add_u32,
+ * 3 v_mul_lo_u32, 4 v_fma_f64.  Gives the denominator of the INT32
+ * multiply-add roofline. */
+int pgx_valu_rate(int device, int op, uint32_t blocks, uint32_t iters, double* per_sec);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGX_H */

@@ -1,0 +1,168 @@
+"""CPU oracle for the batched PRP path.  TEST INFRASTRUCTURE ONLY.
+
+Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, as the checker or as the timed CPU baseline; the product never imports
+it.  Wraps liboracle.so (oracle.c, GMP) and adds pure-Python restatements for
+small inputs (CPython's own bignum ``pow``, independent of GMP).
+
+Reference functions followed (file:line in sethtroisi/prime-gap-gpu):
+  ref_verdict      miller_rabin.h:216-248  (strong test as written, incl. t squarings)
+  sprp             textbook strong test (the reference's intent)
+  residue          BASELINE CPU path: mpz_powm(2, N-1, N)
+  gen_*            gap_common.cpp:60-65 (K = P#/d), gpu_testing.cu:171-172 (N = m*K + x),
+                   gap_search_gpu.cpp:619-689 (sieve of m*K + X by small primes)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built (run make in {_HERE})")
+    lib = ctypes.CDLL(LIB_PATH)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    lib.orc_verdicts.argtypes = [u32p, ctypes.c_size_t, ctypes.c_uint32, u32p, ctypes.c_uint32,
+                                 ctypes.c_uint32, u32p, u32p]
+    lib.orc_probab_prime.argtypes = [u32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int32)]
+    lib.orc_xoshiro_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint32, u64p]
+    lib.orc_gen_multipliers.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                        ctypes.c_uint32, ctypes.c_uint32, u64p]
+    lib.orc_gen_sieved.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u64p, ctypes.c_uint32,
+                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.c_uint32, u32p, u32p, u32p,
+                                   ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_small_factor.argtypes = [u32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
+                                     u32p]
+    lib.orc_time_powm2.argtypes = [u32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int,
+                                   ctypes.c_int, u32p]
+    lib.orc_time_powm2.restype = ctypes.c_double
+    return lib
+
+
+lib = _load()
+
+MODE_REFERENCE, MODE_STRONG, MODE_FERMAT = 0, 1, 2
+
+
+def _u32(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def _u64(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def verdicts(limbs: np.ndarray, mode: int, bases=(2,), residues: bool = False):
+    a = np.ascontiguousarray(limbs, dtype=np.uint32)
+    n, S = a.shape
+    b = np.asarray(bases, dtype=np.uint32)
+    out = np.zeros(n, dtype=np.uint32)
+    res = np.zeros((n, S), dtype=np.uint32) if residues else None
+    rc = lib.orc_verdicts(_u32(a), n, S, _u32(b), len(b), mode, _u32(out),
+                          _u32(res) if residues else None)
+    if rc != 0:
+        raise ValueError(f"orc_verdicts failed: {rc}")
+    return (out, res) if residues else out
+
+
+def probab_prime(limbs: np.ndarray, reps: int = 25) -> np.ndarray:
+    a = np.ascontiguousarray(limbs, dtype=np.uint32)
+    out = np.zeros(a.shape[0], dtype=np.int32)
+    lib.orc_probab_prime(_u32(a), a.shape[0], a.shape[1], reps,
+                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return out
+
+
+def xoshiro_stream(seed: int, count: int) -> np.ndarray:
+    out = np.zeros(count, dtype=np.uint64)
+    lib.orc_xoshiro_stream(seed, count, _u64(out))
+    return out
+
+
+def gen_multipliers(seed: int, m_lo: int, m_hi: int, d: int, n_m: int) -> np.ndarray:
+    out = np.zeros(n_m, dtype=np.uint64)
+    rc = lib.orc_gen_multipliers(seed, m_lo, m_hi, d, n_m, _u64(out))
+    if rc != 0:
+        raise ValueError("orc_gen_multipliers failed")
+    return out
+
+
+def gen_sieved(P, d, m, x_max, per_m, limbs, sieve_limit=1_000_000):
+    m = np.ascontiguousarray(m, dtype=np.uint64)
+    cap = len(m) * per_m
+    out = np.zeros((cap, limbs), dtype=np.uint32)
+    om = np.zeros(cap, dtype=np.uint32)
+    ox = np.zeros(cap, dtype=np.uint32)
+    w = ctypes.c_size_t(0)
+    rc = lib.orc_gen_sieved(P, d, _u64(m), len(m), x_max, per_m, sieve_limit, limbs, _u32(out),
+                            _u32(om), _u32(ox), ctypes.byref(w))
+    if rc != 0:
+        raise ValueError(f"orc_gen_sieved failed: {rc}")
+    n = w.value
+    return out[:n], om[:n], ox[:n]
+
+
+def small_factor(limbs: np.ndarray, limit: int) -> np.ndarray:
+    a = np.ascontiguousarray(limbs, dtype=np.uint32)
+    out = np.zeros(a.shape[0], dtype=np.uint32)
+    lib.orc_small_factor(_u32(a), a.shape[0], a.shape[1], limit, _u32(out))
+    return out
+
+
+def time_powm2(limbs: np.ndarray, threads: int, warmup: bool = True):
+    """(seconds, fermat_count) for one threaded pass of mpz_powm(2, N-1, N)."""
+    a = np.ascontiguousarray(limbs, dtype=np.uint32)
+    cnt = np.zeros(1, dtype=np.uint32)
+    sec = lib.orc_time_powm2(_u32(a), a.shape[0], a.shape[1], threads, int(warmup), _u32(cnt))
+    return sec, int(cnt[0])
+
+
+# ---- pure-Python restatements (CPython bignum; small inputs) ---------------
+
+def py_residue(N: int, a: int = 2) -> int:
+    return pow(a, N - 1, N)
+
+
+def py_ref_verdict(N: int, a: int) -> bool:
+    """miller_rabin.h:221-245 as written."""
+    nm1 = N - 1
+    t = (nm1 & -nm1).bit_length() - 1
+    d = nm1 >> t
+    x = pow(a, d, N)
+    if x == 1 or x == nm1:
+        return True
+    if t == 1:
+        return False
+    count = t
+    while True:
+        x = x * x % N
+        if x == nm1:
+            return True
+        count -= 1
+        if count == 0 or x == 1:
+            return False
+
+
+def py_sprp(N: int, a: int) -> bool:
+    nm1 = N - 1
+    t = (nm1 & -nm1).bit_length() - 1
+    d = nm1 >> t
+    x = pow(a, d, N)
+    if x == 1 or x == nm1:
+        return True
+    for _ in range(t - 1):
+        x = x * x % N
+        if x == nm1:
+            return True
+        if x == 1:
+            return False
+    return False

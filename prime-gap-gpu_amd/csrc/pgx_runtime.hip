@@ -1,0 +1,273 @@
+// C-ABI runtime: runners, staging, launches.  See include/pgx.h for the
+// contract of each entry point and the reference function it replaces.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/pgx.h"
+#include "launch.hpp"
+
+namespace pgx {
+
+PrpLaunchFn* prp_table() {
+  static PrpLaunchFn table[PGX_MAX_KERNEL_LIMBS + 1] = {};
+  return table;
+}
+
+}  // namespace pgx
+
+struct pgx_runner {
+  int device = 0;
+  size_t max_n = 0;
+  uint32_t limbs = 0;
+  pgx::Bases bases{};
+  hipStream_t stream = nullptr;
+  // pinned staging (limb-interleaved, ld = max_n)
+  uint32_t* h_cand = nullptr;
+  uint32_t* h_verdict = nullptr;
+  uint32_t* h_res = nullptr;
+  // device buffers
+  uint32_t* d_cand = nullptr;
+  uint32_t* d_verdict = nullptr;
+  uint32_t* d_res = nullptr;
+  // pending batch
+  bool pending = false;
+  size_t pending_n = 0;
+  uint32_t pending_flags = 0;
+  std::string err;
+};
+
+namespace {
+
+std::string g_create_err = "no error";
+
+int fail(pgx_runner* h, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h)
+    h->err = buf;
+  else
+    g_create_err = buf;
+  return code;
+}
+
+#define PGX_HIP(h, call)                                                                   \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail((h), PGX_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                     \
+  } while (0)
+
+// First `count` primes, the reference's generate_primes (miller_rabin.h:272-289).
+void first_primes(uint32_t count, uint32_t* out) {
+  uint32_t found = 0;
+  for (uint32_t c = 2; found < count; ++c) {
+    bool prime = true;
+    for (uint32_t k = 0; k < found && out[k] * out[k] <= c; ++k)
+      if (c % out[k] == 0) {
+        prime = false;
+        break;
+      }
+    if (prime) out[found++] = c;
+  }
+}
+
+uint32_t bit_length(const uint32_t* x, uint32_t limbs) {
+  for (int j = (int)limbs - 1; j >= 0; --j)
+    if (x[j]) return 32u * j + 32u - __builtin_clz(x[j]);
+  return 0;
+}
+
+void release(pgx_runner* h) {
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->h_cand) (void)hipHostFree(h->h_cand);
+  if (h->h_verdict) (void)hipHostFree(h->h_verdict);
+  if (h->h_res) (void)hipHostFree(h->h_res);
+  if (h->d_cand) (void)hipFree(h->d_cand);
+  if (h->d_verdict) (void)hipFree(h->d_verdict);
+  if (h->d_res) (void)hipFree(h->d_res);
+}
+
+int launch(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs, size_t n,
+           uint32_t max_bits, uint32_t flags, uint32_t* d_verdict, uint32_t* d_res,
+           uint32_t res_ld, hipStream_t stream) {
+  // smallest instantiated kernel limb count that holds max_bits + 2 bits
+  uint32_t S = pgx::kernel_limbs_for_bits(max_bits);
+  while (S <= PGX_MAX_KERNEL_LIMBS && !pgx::prp_table()[S]) ++S;
+  if (S > PGX_MAX_KERNEL_LIMBS)
+    return fail(h, PGX_E_INPUT, "candidates of %u bits exceed the widest kernel (%d limbs)",
+                max_bits, PGX_MAX_KERNEL_LIMBS);
+  pgx::PrpLaunchFn fn = pgx::prp_table()[S];
+  pgx::PrpArgs a{};
+  a.cand = d_cand;
+  a.ld = ld;
+  a.n = (uint32_t)n;
+  a.buf_limbs = limbs;
+  a.mode = flags & PGX_MODE_MASK;
+  a.bases = h->bases;
+  a.verdict = d_verdict;
+  a.residue = (flags & PGX_WANT_RESIDUE) ? d_res : nullptr;
+  a.res_ld = res_ld;
+  PGX_HIP(h, fn(a, stream));
+  return PGX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t limbs,
+                      const uint32_t* bases, uint32_t nbases) {
+  if (!out) return fail(nullptr, PGX_E_ARG, "out is NULL");
+  *out = nullptr;
+  if (max_n == 0 || max_n > 0xFFFFFFFFu)
+    return fail(nullptr, PGX_E_ARG, "max_n must be in [1, 2^32)");
+  if (limbs == 0 || limbs > 4096) return fail(nullptr, PGX_E_ARG, "limbs must be in [1, 4096]");
+  if (nbases == 0 || nbases > PGX_MAX_BASES)
+    return fail(nullptr, PGX_E_ARG, "nbases must be in [1, %d]", PGX_MAX_BASES);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, PGX_E_NODEV, "no HIP device available");
+  if (device < 0 || device >= ndev)
+    return fail(nullptr, PGX_E_ARG, "device %d out of range (%d devices)", device, ndev);
+
+  pgx_runner* h = new (std::nothrow) pgx_runner();
+  if (!h) return fail(nullptr, PGX_E_ARG, "out of host memory");
+  h->device = device;
+  h->max_n = max_n;
+  h->limbs = limbs;
+  h->bases.count = nbases;
+  if (bases) {
+    for (uint32_t k = 0; k < nbases; ++k) {
+      if (bases[k] < 2) {
+        delete h;
+        return fail(nullptr, PGX_E_ARG, "base %u must be >= 2", bases[k]);
+      }
+      h->bases.a[k] = bases[k];
+    }
+  } else {
+    first_primes(nbases, h->bases.a);
+  }
+  const size_t words = (size_t)limbs * max_n;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_cand, words * 4, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_verdict, max_n * 4, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_res, words * 4, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void**)&h->d_cand, words * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&h->d_verdict, max_n * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&h->d_res, words * 4);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    release(h);
+    delete h;
+    return fail(nullptr, PGX_E_HIP, "runner allocation failed: %s", hipGetErrorString(e));
+  }
+  h->err = "no error";
+  *out = h;
+  return PGX_OK;
+}
+
+int pgx_runner_destroy(pgx_runner_t* h) {
+  if (!h) return PGX_E_ARG;
+  (void)hipSetDevice(h->device);
+  if (h->pending) (void)hipStreamSynchronize(h->stream);
+  release(h);
+  delete h;
+  return PGX_OK;
+}
+
+int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t flags) {
+  if (!h) return PGX_E_ARG;
+  if (h->pending) return fail(h, PGX_E_STATE, "a batch is already in flight");
+  if (n == 0) {  // the reference returns early on an empty batch (miller_rabin.h:365-366)
+    h->pending = true;
+    h->pending_n = 0;
+    h->pending_flags = flags;
+    return PGX_OK;
+  }
+  if (!limbs_le) return fail(h, PGX_E_ARG, "limbs_le is NULL");
+  if (n > h->max_n)
+    return fail(h, PGX_E_CAPACITY, "tests(%zu) > n(%zu)", n, h->max_n);
+  if ((flags & PGX_MODE_MASK) > PGX_MODE_FERMAT) return fail(h, PGX_E_ARG, "bad mode");
+  const uint32_t S = h->limbs;
+  uint32_t max_bits = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const uint32_t* x = limbs_le + i * S;
+    const uint32_t L = bit_length(x, S);
+    if (!(x[0] & 1u) || L < 2)
+      return fail(h, PGX_E_INPUT, "candidate %zu is even or <= 1", i);
+    if (L + 2 > 32u * PGX_MAX_KERNEL_LIMBS)
+      return fail(h, PGX_E_INPUT, "candidate %zu has %u bits, more than the kernels support", i, L);
+    max_bits = std::max(max_bits, L);
+    for (uint32_t j = 0; j < S; ++j) h->h_cand[(size_t)j * n + i] = x[j];
+  }
+  PGX_HIP(h, hipSetDevice(h->device));
+  PGX_HIP(h, hipMemcpyAsync(h->d_cand, h->h_cand, (size_t)S * n * 4, hipMemcpyHostToDevice,
+                            h->stream));
+  int rc = launch(h, h->d_cand, (uint32_t)n, S, n, max_bits, flags, h->d_verdict, h->d_res,
+                  (uint32_t)n, h->stream);
+  if (rc != PGX_OK) return rc;
+  PGX_HIP(h, hipMemcpyAsync(h->h_verdict, h->d_verdict, n * 4, hipMemcpyDeviceToHost, h->stream));
+  if (flags & PGX_WANT_RESIDUE)
+    PGX_HIP(h, hipMemcpyAsync(h->h_res, h->d_res, (size_t)S * n * 4, hipMemcpyDeviceToHost,
+                              h->stream));
+  h->pending = true;
+  h->pending_n = n;
+  h->pending_flags = flags;
+  return PGX_OK;
+}
+
+int pgx_wait(pgx_runner_t* h, uint32_t* verdict, uint32_t* residues) {
+  if (!h) return PGX_E_ARG;
+  if (!h->pending) return fail(h, PGX_E_STATE, "no batch in flight");
+  h->pending = false;
+  const size_t n = h->pending_n;
+  if (n == 0) return PGX_OK;
+  PGX_HIP(h, hipStreamSynchronize(h->stream));
+  if (verdict) std::memcpy(verdict, h->h_verdict, n * 4);
+  if ((h->pending_flags & PGX_WANT_RESIDUE) && residues) {
+    const uint32_t S = h->limbs;
+    for (size_t i = 0; i < n; ++i)
+      for (uint32_t j = 0; j < S; ++j) residues[i * S + j] = h->h_res[(size_t)j * n + i];
+  }
+  return PGX_OK;
+}
+
+int pgx_run(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t flags,
+            uint32_t* verdict, uint32_t* residues) {
+  int rc = pgx_submit(h, limbs_le, n, flags);
+  if (rc != PGX_OK) return rc;
+  return pgx_wait(h, verdict, residues);
+}
+
+int pgx_test_device(pgx_runner_t* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs,
+                    size_t n, uint32_t max_bits, uint32_t flags, uint32_t* d_verdict,
+                    uint32_t* d_residues, uint32_t res_ld, void* stream) {
+  if (!h) return PGX_E_ARG;
+  if (n == 0) return PGX_OK;
+  if (!d_cand || !d_verdict) return fail(h, PGX_E_ARG, "null device pointer");
+  if (n > 0xFFFFFFFFu || ld < n || limbs == 0)
+    return fail(h, PGX_E_ARG, "bad shape: n=%zu ld=%u limbs=%u", n, ld, limbs);
+  if ((flags & PGX_WANT_RESIDUE) && (!d_residues || res_ld < n))
+    return fail(h, PGX_E_ARG, "residues requested without a buffer of res_ld >= n");
+  if ((flags & PGX_MODE_MASK) > PGX_MODE_FERMAT) return fail(h, PGX_E_ARG, "bad mode");
+  if (max_bits < 2) return fail(h, PGX_E_ARG, "max_bits must be >= 2");
+  PGX_HIP(h, hipSetDevice(h->device));
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  return launch(h, d_cand, ld, limbs, n, max_bits, flags, d_verdict, d_residues, res_ld, s);
+}
+
+const char* pgx_last_error(const pgx_runner_t* h) {
+  return h ? h->err.c_str() : g_create_err.c_str();
+}
+
+}  // extern "C"

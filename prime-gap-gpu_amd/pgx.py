@@ -1,0 +1,269 @@
+"""Python binding of the gfx950 PRP tester's C ABI (include/pgx.h).
+
+Host-side mirror of the reference's runner interface, test_runner_t
+(miller_rabin.h:295-409): ``Runner(n, rounds)`` owns the pinned staging,
+device buffers and stream; ``Runner.run_test(active, tests, results)`` has the
+reference's semantics (tests = Python ints, results[i] = 1 when every base
+passed, entries from ``active`` on left untouched).  ``Runner.run`` exposes the
+raw verdict words and residues on numpy limb arrays, and ``Runner.test_device``
+launches on device-resident, limb-interleaved buffers (torch tensors or raw
+pointers) without any host copy.
+
+The library is libpgx.so next to this file, built by ``make`` in this
+directory.  There is no CPU fallback: importing this module without the built
+library raises ImportError, and creating a runner without a GPU raises
+PgxError.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpgx.so")
+
+PGX_OK = 0
+PGX_E_ARG = -1
+PGX_E_HIP = -2
+PGX_E_CAPACITY = -3
+PGX_E_INPUT = -4
+PGX_E_STATE = -5
+PGX_E_NODEV = -6
+
+MODE_REFERENCE = 0
+MODE_STRONG = 1
+MODE_FERMAT = 2
+WANT_RESIDUE = 4
+
+FLAG_FERMAT = 1
+FLAG_SPRP = 2
+FLAG_QUIRK = 4
+VERDICT_INVALID = 0xFFFFFFFF
+MAX_KERNEL_LIMBS = 64
+
+EXPORTED = (
+    "pgx_runner_create",
+    "pgx_runner_destroy",
+    "pgx_submit",
+    "pgx_wait",
+    "pgx_run",
+    "pgx_test_device",
+    "pgx_last_error",
+    "pgx_gen_sieved",
+    "pgx_gen_multipliers",
+    "pgx_k_bits",
+    "pgx_valu_rate",
+)
+
+
+class PgxError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"pgx error {code}: {message}")
+        self.code = code
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built (run make in {_HERE})")
+    lib = ctypes.CDLL(LIB_PATH)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    vp = ctypes.c_void_p
+    lib.pgx_runner_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_size_t,
+                                      ctypes.c_uint32, u32p, ctypes.c_uint32]
+    lib.pgx_runner_destroy.argtypes = [vp]
+    lib.pgx_submit.argtypes = [vp, u32p, ctypes.c_size_t, ctypes.c_uint32]
+    lib.pgx_wait.argtypes = [vp, u32p, u32p]
+    lib.pgx_run.argtypes = [vp, u32p, ctypes.c_size_t, ctypes.c_uint32, u32p, u32p]
+    lib.pgx_test_device.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_size_t,
+                                    ctypes.c_uint32, ctypes.c_uint32, vp, vp, ctypes.c_uint32, vp]
+    lib.pgx_last_error.argtypes = [vp]
+    lib.pgx_last_error.restype = ctypes.c_char_p
+    lib.pgx_gen_sieved.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u64p, ctypes.c_uint32,
+                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.c_uint32, u32p, u32p, u32p,
+                                   ctypes.POINTER(ctypes.c_size_t)]
+    lib.pgx_gen_multipliers.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                        ctypes.c_uint32, ctypes.c_uint32, u64p]
+    lib.pgx_k_bits.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    lib.pgx_k_bits.restype = ctypes.c_uint32
+    lib.pgx_valu_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                  ctypes.POINTER(ctypes.c_double)]
+    return lib
+
+
+lib = _load()
+
+
+def _u32(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def _u64(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def first_primes(count: int) -> list[int]:
+    """The reference's generate_primes(count) (miller_rabin.h:272-289)."""
+    out: list[int] = []
+    c = 2
+    while len(out) < count:
+        if all(c % p for p in out if p * p <= c):
+            out.append(c)
+        c += 1
+    return out
+
+
+# ---- limb packing (from_mpz / to_mpz, miller_rabin.h:39-54) ---------------
+
+def to_limbs(values, limbs: int) -> np.ndarray:
+    """Candidate-major little-endian u32 limbs; raises if a value does not fit
+    (the reference's from_mpz exits instead, miller_rabin.h:46-49)."""
+    values = list(values)
+    out = np.zeros((len(values), limbs), dtype=np.uint32)
+    for i, v in enumerate(values):
+        if v < 0 or v.bit_length() > 32 * limbs:
+            raise PgxError(PGX_E_INPUT, f"candidate {i} does not fit in {limbs} limbs")
+        b = v.to_bytes(4 * limbs, "little")
+        out[i] = np.frombuffer(b, dtype="<u4")
+    return out
+
+
+def from_limbs(arr: np.ndarray) -> list[int]:
+    arr = np.ascontiguousarray(arr, dtype="<u4")
+    return [int.from_bytes(row.tobytes(), "little") for row in arr]
+
+
+# ---- candidate builder (fill_batch / init_K roles) -----------------------
+
+def k_bits(P: int, d: int) -> int:
+    return int(lib.pgx_k_bits(P, d))
+
+
+def gen_multipliers(seed: int, m_lo: int, m_hi: int, d: int, n_m: int) -> np.ndarray:
+    out = np.zeros(n_m, dtype=np.uint64)
+    rc = lib.pgx_gen_multipliers(seed, m_lo, m_hi, d, n_m, _u64(out))
+    if rc != PGX_OK:
+        raise PgxError(rc, "pgx_gen_multipliers failed (not enough multipliers coprime to d?)")
+    return out
+
+
+def gen_sieved(P: int, d: int, m: np.ndarray, x_max: int, per_m: int, limbs: int,
+               sieve_limit: int = 1_000_000):
+    """Sieved candidates N = m*K + X: returns (limbs[count, limbs], m_index, X)."""
+    m = np.ascontiguousarray(m, dtype=np.uint64)
+    cap = len(m) * per_m
+    out = np.zeros((cap, limbs), dtype=np.uint32)
+    om = np.zeros(cap, dtype=np.uint32)
+    ox = np.zeros(cap, dtype=np.uint32)
+    w = ctypes.c_size_t(0)
+    rc = lib.pgx_gen_sieved(P, d, _u64(m), len(m), x_max, per_m, sieve_limit, limbs,
+                            _u32(out), _u32(om), _u32(ox), ctypes.byref(w))
+    if rc != PGX_OK:
+        raise PgxError(rc, "pgx_gen_sieved failed (d must divide P#, N must fit in limbs)")
+    n = w.value
+    return out[:n], om[:n], ox[:n]
+
+
+VALU_OPS = {0: "v_mad_u64_u32", 1: "mad+addc", 2: "v_add_u32", 3: "v_mul_lo_u32",
+            4: "v_fma_f64"}
+
+
+def valu_rate(op: int, device: int = 0, blocks: int = 8192, iters: int = 4096) -> float:
+    """Lane-instructions per second of VALU op `op` (see pgx_valu_rate)."""
+    out = ctypes.c_double(0)
+    rc = lib.pgx_valu_rate(device, op, blocks, iters, ctypes.byref(out))
+    if rc != PGX_OK:
+        raise PgxError(rc, f"pgx_valu_rate({op}) failed")
+    return out.value
+
+
+# ---- runner ---------------------------------------------------------------
+
+class Runner:
+    """One runner per host thread, like test_runner_t (miller_rabin.h:295-409)."""
+
+    def __init__(self, n: int, rounds: int = 1, limbs: int = 16, bases=None, device: int = 0):
+        self.n = int(n)
+        self.limbs = int(limbs)
+        if bases is None:
+            bases = first_primes(rounds)
+        self.bases = [int(b) for b in bases]
+        self.rounds = len(self.bases)
+        arr = np.asarray(self.bases, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        rc = lib.pgx_runner_create(ctypes.byref(h), device, self.n, self.limbs, _u32(arr),
+                                   len(self.bases))
+        if rc != PGX_OK:
+            raise PgxError(rc, lib.pgx_last_error(None).decode())
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.pgx_runner_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != PGX_OK:
+            raise PgxError(rc, lib.pgx_last_error(self._h).decode())
+
+    def run(self, limbs_arr: np.ndarray, mode: int = MODE_REFERENCE, residues: bool = False):
+        """Verdict words (and residues as [n, limbs] u32) for candidate-major limbs."""
+        a = np.ascontiguousarray(limbs_arr, dtype=np.uint32)
+        if a.ndim != 2 or a.shape[1] != self.limbs:
+            raise PgxError(PGX_E_ARG, f"expected [n, {self.limbs}] limbs, got {a.shape}")
+        n = a.shape[0]
+        v = np.zeros(n, dtype=np.uint32)
+        r = np.zeros((n, self.limbs), dtype=np.uint32) if residues else None
+        flags = mode | (WANT_RESIDUE if residues else 0)
+        self._check(lib.pgx_run(self._h, _u32(a), n, flags, _u32(v),
+                                _u32(r) if residues else None))
+        return (v, r) if residues else v
+
+    def submit(self, limbs_arr: np.ndarray, mode: int = MODE_REFERENCE, residues: bool = False):
+        a = np.ascontiguousarray(limbs_arr, dtype=np.uint32)
+        self._pending = (a.shape[0], residues, a)
+        flags = mode | (WANT_RESIDUE if residues else 0)
+        self._check(lib.pgx_submit(self._h, _u32(a), a.shape[0], flags))
+
+    def wait(self):
+        n, residues, _ = self._pending
+        v = np.zeros(n, dtype=np.uint32)
+        r = np.zeros((n, self.limbs), dtype=np.uint32) if residues else None
+        self._check(lib.pgx_wait(self._h, _u32(v), _u32(r) if residues else None))
+        self._pending = None
+        return (v, r) if residues else v
+
+    def run_test(self, active: int, tests, results) -> None:
+        """Reference semantics of run_test (miller_rabin.h:364-408): tests is the
+        full batch of ints; the first min(active, len(tests)) are tested and
+        results[i] = (passed == rounds) for those."""
+        if len(tests) == 0:
+            return
+        if len(tests) > self.n:
+            raise PgxError(PGX_E_CAPACITY, f"tests({len(tests)}) > n({self.n})")
+        to_run = min(active, len(tests))
+        v = self.run(to_limbs(tests[:to_run], self.limbs), MODE_REFERENCE)
+        for i in range(to_run):
+            results[i] = int(v[i] == self.rounds)
+
+    def test_device(self, cand_ptr: int, ld: int, limbs: int, n: int, max_bits: int,
+                    mode: int, verdict_ptr: int, residue_ptr: int = 0, res_ld: int = 0,
+                    stream: int = 0) -> None:
+        """Launch on device-resident limb-interleaved buffers (raw pointers)."""
+        flags = mode | (WANT_RESIDUE if residue_ptr else 0)
+        self._check(lib.pgx_test_device(self._h, ctypes.c_void_p(cand_ptr), ld, limbs, n,
+                                        max_bits, flags, ctypes.c_void_p(verdict_ptr),
+                                        ctypes.c_void_p(residue_ptr or None), res_ld,
+                                        ctypes.c_void_p(stream or None)))

@@ -1,0 +1,184 @@
+"""GPU parity: the gfx950 kernels, called through the C ABI, against the golden
+vectors and the GMP oracle on the same inputs.  Bit-exact: every verdict word
+and every residue 2^(N-1) mod N."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pgx
+from conftest import limbs_needed
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def expect_counts(recs, key, bases):
+    out = []
+    for r in recs:
+        c = 0
+        for a in bases:
+            if not r[key][str(a)]:
+                break
+            c += 1
+        out.append(c)
+    return out
+
+
+def check_records(recs, limbs, label):
+    ns = [int(r["n"], 16) for r in recs]
+    L = pgx.to_limbs(ns, limbs)
+    say(f"{label}: {len(ns)} records, {limbs} limbs, fermat+residues")
+    with pgx.Runner(len(ns), rounds=4, limbs=limbs) as run:
+        v, res = run.run(L, pgx.MODE_FERMAT, residues=True)
+        assert pgx.from_limbs(res) == [int(r["res2"], 16) for r in recs]
+        assert [bool(x & 1) for x in v] == [r["fermat2"] for r in recs]
+        assert [bool(x & 2) for x in v] == [r["sprp"]["2"] for r in recs]
+        assert [bool(x & 4) for x in v] == [r["ref"]["2"] for r in recs]
+        say(f"{label}: reference + strong modes, bases 2,3,5,7")
+        ref = run.run(L, pgx.MODE_REFERENCE)
+        assert list(ref) == expect_counts(recs, "ref", (2, 3, 5, 7))
+        st = run.run(L, pgx.MODE_STRONG)
+        assert list(st) == expect_counts(recs, "sprp", (2, 3, 5, 7))
+
+
+def test_kat_vectors(vectors):
+    recs = vectors["kat"]
+    check_records(recs, max(limbs_needed(int(r["n"], 16)) for r in recs), "kat")
+
+
+def test_every_kernel_width(vectors):
+    # one runner per stored width so each kernel limb count S is selected
+    by_limbs = {}
+    for r in vectors["widths"]:
+        by_limbs.setdefault(limbs_needed(int(r["n"], 16)), []).append(r)
+    for limbs, recs in sorted(by_limbs.items()):
+        check_records(recs, limbs, f"widths[{limbs}]")
+
+
+def test_sieved_fixture_sets(vectors):
+    for name, s in vectors["sieved"].items():
+        check_records(s["cands"], s["limbs"], name)
+
+
+def test_tiny_brute_force(vectors):
+    t = vectors["tiny_res2"]
+    ns = list(range(t["first"], t["first"] + t["step"] * len(t["res"]), t["step"]))
+    say(f"tiny: {len(ns)} odd N < 4096")
+    with pgx.Runner(len(ns), limbs=1) as run:
+        _, res = run.run(pgx.to_limbs(ns, 1), pgx.MODE_FERMAT, residues=True)
+    assert [int(x) for x in res[:, 0]] == [int(h, 16) for h in t["res"]]
+
+
+def test_all_odd_below_2_16_vs_oracle():
+    ns = list(range(3, 1 << 16, 2))
+    L = pgx.to_limbs(ns, 1)
+    say(f"odd N < 2^16: {len(ns)} candidates, all modes")
+    ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+    with pgx.Runner(len(ns), rounds=3, limbs=1) as run:
+        v, res = run.run(L, pgx.MODE_FERMAT, residues=True)
+        assert np.array_equal(v, ov) and np.array_equal(res, orr)
+        for mode in (pgx.MODE_REFERENCE, pgx.MODE_STRONG):
+            assert np.array_equal(run.run(L, mode), O.verdicts(L, mode, (2, 3, 5)))
+
+
+@pytest.mark.parametrize("P,d,limbs,per_m", [(337, 2310, 16, 1024), (211, 210, 16, 256),
+                                             (503, 210, 24, 256), (701, 210, 32, 256),
+                                             (907, 30030, 64, 128)])
+def test_sieved_candidates_vs_gmp(P, d, limbs, per_m):
+    """Configs' generators at reduced counts: residues and verdicts vs mpz_powm."""
+    m = pgx.gen_multipliers(0x5EED0000 + P, 10**7, 10**7 + 10**4, d, 16)
+    L, _, _ = pgx.gen_sieved(P, d, m, 450000, per_m, limbs)
+    say(f"sieved P={P} d={d}: {len(L)} candidates in {limbs} limbs")
+    ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+    with pgx.Runner(len(L), rounds=3, limbs=limbs) as run:
+        v, res = run.run(L, pgx.MODE_FERMAT, residues=True)
+        assert np.array_equal(res, orr)
+        assert np.array_equal(v, ov)
+        assert np.array_equal(run.run(L, pgx.MODE_REFERENCE),
+                              O.verdicts(L, O.MODE_REFERENCE, (2, 3, 5)))
+    # the reference's verdict equals the strong test's on every input
+    assert np.array_equal((v >> 1) & 1, (v >> 2) & 1)
+    assert (v & 1).sum() > 0
+
+
+def test_config1_16384_candidates_vs_gmp():
+    """P=337, d=2310, 512-bit buffers, 16384 sieved candidates (X widened)."""
+    m = pgx.gen_multipliers(0x5EED0001, 10**7, 10**7 + 10**4, 2310, 16)
+    L, _, _ = pgx.gen_sieved(337, 2310, m, 450000, 1024, 16)
+    assert len(L) == 16384
+    say("config-1 set: 16384 candidates")
+    ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+    with pgx.Runner(16384, limbs=16) as run:
+        v, res = run.run(L, pgx.MODE_FERMAT, residues=True)
+    assert np.array_equal(v, ov)
+    assert np.array_equal(res, orr)
+
+
+def test_run_test_reference_semantics():
+    """run_test(active, tests, results): only the first `active` entries are
+    tested and written; results = (passed == rounds) (miller_rabin.h:364-408)."""
+    primes = [2**61 - 1, 2**89 - 1, 2**107 - 1, 1000003]
+    comps = [3215031751, 2047, 561, 2**61 + 1]
+    tests = primes + comps
+    with pgx.Runner(16, rounds=1, limbs=4) as run:
+        res = [-1] * len(tests)
+        run.run_test(6, tests, res)
+        # 3215031751 and 2047 are base-2 strong pseudoprimes
+        assert res == [1, 1, 1, 1, 1, 1, -1, -1]
+    with pgx.Runner(16, rounds=4, limbs=4) as run:
+        res = [-1] * len(tests)
+        run.run_test(len(tests), tests, res)
+        # 3215031751 passes bases 2,3,5,7 (it is a strong pseudoprime to all four)
+        assert res == [1, 1, 1, 1, 1, 0, 0, 0]
+        run.run_test(0, tests, res)  # nothing tested
+        run.run_test(3, [], res)     # empty batch returns early
+
+
+def test_submit_wait_and_errors():
+    with pgx.Runner(8, limbs=2) as run:
+        L = pgx.to_limbs([7, 9, 11], 2)
+        run.submit(L, pgx.MODE_FERMAT, residues=True)
+        with pytest.raises(pgx.PgxError) as e:
+            run.submit(L, pgx.MODE_FERMAT)
+        assert e.value.code == pgx.PGX_E_STATE
+        v, r = run.wait()
+        assert list(v & 1) == [1, 0, 1] and pgx.from_limbs(r) == [1, 4, 1]
+        for bad in ([8], [1], [0]):
+            with pytest.raises(pgx.PgxError) as e:
+                run.run(pgx.to_limbs(bad, 2), pgx.MODE_FERMAT)
+            assert e.value.code == pgx.PGX_E_INPUT
+        with pytest.raises(pgx.PgxError) as e:
+            run.run(pgx.to_limbs([7] * 9, 2), pgx.MODE_FERMAT)
+        assert e.value.code == pgx.PGX_E_CAPACITY
+        # the runner still works after rejected calls
+        assert list(run.run(pgx.to_limbs([13], 2), pgx.MODE_FERMAT) & 1) == [1]
+
+
+def test_device_resident_interleaved_layout():
+    """pgx_test_device on torch-owned HBM buffers, limb-interleaved, ld > n."""
+    import torch
+    m = pgx.gen_multipliers(0x5EED0002, 10**7, 10**7 + 10**4, 2310, 4)
+    L, _, _ = pgx.gen_sieved(337, 2310, m, 450000, 1000, 16)
+    n, limbs = L.shape
+    ld = n + 64
+    say(f"device-resident: n={n} ld={ld}")
+    host = np.zeros((limbs, ld), dtype=np.uint32)
+    host[:, :n] = L.T
+    dev = torch.from_numpy(host.view(np.int32)).cuda()
+    verdict = torch.zeros(n, dtype=torch.int32, device="cuda")
+    res = torch.zeros((limbs, ld), dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    with pgx.Runner(1, limbs=limbs) as run:
+        run.test_device(dev.data_ptr(), ld, limbs, n, 462, pgx.MODE_FERMAT, verdict.data_ptr(),
+                        res.data_ptr(), ld, stream)
+        torch.cuda.synchronize()
+    v = verdict.cpu().numpy().view(np.uint32)
+    r = res.cpu().numpy().view(np.uint32)[:, :n].T
+    ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+    assert np.array_equal(v, ov) and np.array_equal(r, orr)

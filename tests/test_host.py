@@ -1,0 +1,151 @@
+"""CPU-side tests of the product's host code: candidate builder vs the oracle,
+limb packing, the C ABI's exported symbols, loud failure without a GPU, and the
+host build of the kernels' per-lane routine vs the oracle."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pgx
+from conftest import ROOT, limbs_needed
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "pgx.h")).read()
+    declared = set(re.findall(r"\b(pgx_[a-z_]+)\s*\(", hdr))
+    lib = ctypes.CDLL(pgx.LIB_PATH)
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert declared >= set(pgx.EXPORTED)
+
+
+def test_first_primes_matches_generate_primes():
+    # miller_rabin.h:272-289: list[0] = 2, then odd numbers not divisible by an
+    # earlier odd prime
+    assert pgx.first_primes(10) == [2, 3, 5, 7, 11, 13, 17, 19, 23, 29]
+
+
+def test_limb_packing_round_trip():
+    rng = np.random.default_rng(1)
+    vals = [int(x) for x in rng.integers(1, 1 << 62, size=50)] + [2**511 + 1, 2**512 - 1, 1]
+    arr = pgx.to_limbs(vals, 16)
+    assert pgx.from_limbs(arr) == vals
+    # same layout as mpz_export(order=-1, size=4): limb 0 is the low word
+    assert arr[-3, 0] == 1 and arr[-3, 15] == 0x80000000
+    with pytest.raises(pgx.PgxError):
+        pgx.to_limbs([2**512], 16)
+
+
+@pytest.mark.parametrize("P,d,limbs", [(337, 2310, 16), (211, 210, 16), (503, 210, 24),
+                                       (701, 210, 32), (907, 30030, 64)])
+def test_generator_matches_oracle(P, d, limbs):
+    seed = 0x5EED0000 + P
+    m = pgx.gen_multipliers(seed, 10**7, 10**7 + 10**4, d, 3)
+    mo = O.gen_multipliers(seed, 10**7, 10**7 + 10**4, d, 3)
+    assert np.array_equal(m, mo)
+    a = pgx.gen_sieved(P, d, m, 30000, 300, limbs)
+    b = O.gen_sieved(P, d, mo, 30000, 300, limbs)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert len(a[0]) > 0
+
+
+def test_generator_rejects_bad_input():
+    assert pgx.k_bits(337, 7 * 1009) == 0  # 1009 does not divide 337#
+    with pytest.raises(pgx.PgxError):
+        pgx.gen_sieved(337, 1009, np.array([10**7 + 1], dtype=np.uint64), 100, 10, 16)
+    with pytest.raises(pgx.PgxError):  # N (~462 bits) does not fit 8 limbs
+        pgx.gen_sieved(337, 2310, np.array([10**7 + 1], dtype=np.uint64), 1000, 10, 8)
+    with pytest.raises(pgx.PgxError):  # not enough coprime multipliers
+        pgx.gen_multipliers(1, 10, 12, 2310, 5)
+
+
+def test_k_bits_matches_survey_table():
+    assert pgx.k_bits(337, 2310) == 439
+    assert pgx.k_bits(907, 30030) == 1236
+
+
+def test_runner_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(pgx.PgxError) as e:
+        pgx.Runner(16, rounds=1, limbs=16)
+    assert e.value.code == pgx.PGX_E_NODEV
+
+
+# ---- host build of the per-lane kernel routine -----------------------------
+
+EMU_PATH = os.path.join(ROOT, "prime-gap-gpu_amd", "libpgx_lane_emu.so")
+
+
+@pytest.fixture(scope="module")
+def emu():
+    if not os.path.exists(EMU_PATH):
+        pytest.fail("libpgx_lane_emu.so is not built")
+    lib = ctypes.CDLL(EMU_PATH)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    lib.pgx_lane_emu.argtypes = [u32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p]
+    lib.pgx_lane_emu.restype = ctypes.c_uint32
+
+    def run(L, S, mode, bases=(2,)):
+        L = np.ascontiguousarray(L, dtype=np.uint32)
+        n, limbs = L.shape
+        v = np.zeros(n, np.uint32)
+        r = np.zeros_like(L)
+        b = np.asarray(bases, np.uint32)
+        s = lib.pgx_lane_emu(pgx._u32(L), n, limbs, S, mode, pgx._u32(b), len(b), pgx._u32(v),
+                             pgx._u32(r))
+        assert s > 0
+        return v, r
+    return run
+
+
+def test_lane_routine_tiny_brute_force(emu):
+    L = pgx.to_limbs(range(3, 1 << 13, 2), 1)
+    for S in (1, 2, 3):
+        v, r = emu(L, S, pgx.MODE_FERMAT)
+        ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+        assert np.array_equal(v, ov) and np.array_equal(r, orr)
+
+
+def test_lane_routine_fixture_records(emu, vectors):
+    recs = [r for r in vectors["kat"] + vectors["widths"] if int(r["n"], 16).bit_length() <= 734]
+    ns = [int(r["n"], 16) for r in recs]
+    L = pgx.to_limbs(ns, 24)
+    v, r = emu(L, 23, pgx.MODE_FERMAT)
+    assert pgx.from_limbs(r) == [int(x["res2"], 16) for x in recs]
+    for i, rec in enumerate(recs):
+        assert bool(v[i] & 1) == rec["fermat2"] and bool(v[i] & 2) == rec["sprp"]["2"]
+    for mode, key in ((pgx.MODE_REFERENCE, "ref"), (pgx.MODE_STRONG, "sprp")):
+        v, _ = emu(L, 23, mode, (2, 3, 5, 7))
+        for i, rec in enumerate(recs):
+            want = 0
+            for a in "2357":
+                if not rec[key][a]:
+                    break
+                want += 1
+            assert v[i] == want
+
+
+def test_lane_routine_sieved_vs_oracle(emu):
+    m = pgx.gen_multipliers(0x5EED0001, 10**7, 10**7 + 10**4, 2310, 2)
+    L, _, _ = pgx.gen_sieved(337, 2310, m, 200000, 200, 16)
+    for S in (15, 16):
+        v, r = emu(L, S, pgx.MODE_FERMAT)
+        ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+        assert np.array_equal(v, ov) and np.array_equal(r, orr)
+    v, _ = emu(L, 15, pgx.MODE_REFERENCE, (2, 3, 5))
+    assert np.array_equal(v, O.verdicts(L, O.MODE_REFERENCE, (2, 3, 5)))
+
+
+def test_lane_routine_rejects_what_does_not_fit(emu):
+    # even, one, and a value too wide for S=1 (needs 4N < 2^32)
+    L = pgx.to_limbs([10, 1, (1 << 31) - 1, (1 << 30) - 1], 1)
+    v, _ = emu(L, 1, pgx.MODE_FERMAT)
+    assert list(v[:3]) == [pgx.VERDICT_INVALID] * 3
+    assert v[3] != pgx.VERDICT_INVALID
