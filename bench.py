@@ -102,7 +102,9 @@ def main() -> int:
     dev = torch.from_numpy(host.view(np.int32)).cuda()
     verdict = torch.zeros((nb, BATCH), dtype=torch.int32, device="cuda")
     res = torch.zeros((nb, LIMBS, BATCH), dtype=torch.int32, device="cuda")
-    stream = torch.cuda.current_stream()
+    # a dedicated stream (default priority): kernels and events share it; a
+    # NULL stream handle would mean the runner's own stream to the library
+    stream = torch.cuda.Stream()
     runner = pgx.Runner(1, limbs=LIMBS)
 
     def step(events=None):

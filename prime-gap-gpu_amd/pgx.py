@@ -13,6 +13,13 @@ The library is libpgx.so next to this file, built by ``make`` in this
 directory.  There is no CPU fallback: importing this module without the built
 library raises ImportError, and creating a runner without a GPU raises
 PgxError.
+
+One HIP runtime per process: PyTorch ships its own libamdhip64 (same soname
+as /opt/rocm's, but its own libraries request it by another name), so torch is
+imported before libpgx.so is loaded.  libpgx.so's libamdhip64.so.7 dependency
+then binds to torch's copy and device pointers, streams and events are shared.
+Loading libpgx.so first would bring in a second runtime that hides the GPU
+from torch.
 """
 from __future__ import annotations
 
@@ -20,6 +27,11 @@ import ctypes
 import os
 
 import numpy as np
+
+try:  # bind libpgx.so to the HIP runtime torch loads (see above)
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - torch is part of the image
+    torch = None
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpgx.so")
@@ -233,9 +245,9 @@ class Runner:
 
     def submit(self, limbs_arr: np.ndarray, mode: int = MODE_REFERENCE, residues: bool = False):
         a = np.ascontiguousarray(limbs_arr, dtype=np.uint32)
-        self._pending = (a.shape[0], residues, a)
         flags = mode | (WANT_RESIDUE if residues else 0)
         self._check(lib.pgx_submit(self._h, _u32(a), a.shape[0], flags))
+        self._pending = (a.shape[0], residues, a)  # only once the library accepted it
 
     def wait(self):
         n, residues, _ = self._pending
