@@ -47,6 +47,15 @@ def work_per_test(bits: int) -> int:
     return (bits - 1) * (2 * s * s + s)
 
 
+def kernel_limbs(max_bits: int) -> int:
+    """Radix limbs of the kernel the library picks: the smallest S with
+    RB*S - 4 >= max_bits (RB = 29 up to 31 limbs, 28 above)."""
+    S = 1
+    while (29 if S <= 31 else 28) * S - 4 < max_bits:
+        S += 1
+    return S
+
+
 def cpu_baseline(cands: np.ndarray, target_cpu_s: float = 15.0) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the GMP path, timed as the reported CPU baseline
@@ -180,7 +189,7 @@ def main() -> int:
                         "primes<1e6 sieved), 16-limb buffers, 16 launches x 65536, base-2 Fermat "
                         "+ residues, plain one-launch-per-batch kernel",
             "bits": L,
-            "kernel_limbs": (max_bits + 2 + 31) // 32,
+            "kernel_limbs": kernel_limbs(max_bits),
             "global_batch": total,
             "launch_batch": BATCH,
         },
@@ -191,7 +200,7 @@ def main() -> int:
             "unit": "TMAC/s",
             "frac": achieved / peak if peak else None,
             "traffic": None,
-            "kernel": "k_prp_plain<15,true>",
+            "kernel": f"k_prp_plain<{kernel_limbs(max_bits)},true>",
             "macs_per_launch": macs_launch,
             "avg_launch_ms": avg_launch_ms,
             "valu_rates_per_s": rate,

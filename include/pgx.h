@@ -64,7 +64,7 @@ extern "C" {
 
 #define PGX_VERDICT_INVALID 0xFFFFFFFFu /* candidate rejected by the kernel */
 #define PGX_MAX_BASES 16
-#define PGX_MAX_KERNEL_LIMBS 64 /* N < 2^(32*64 - 2) */
+#define PGX_MAX_BITS 2068 /* widest candidate the kernels take (74 radix-2^28 limbs) */
 
 typedef struct pgx_runner pgx_runner_t;
 
@@ -138,8 +138,8 @@ uint32_t pgx_k_bits(uint32_t P, uint32_t d);
  * VALU issue-rate microbenchmark (no counterpart in the reference, which
  * publishes no roofline): lane-instructions per second of one instruction
  * kind on `device`, 8 independent chains per lane, `blocks` x 256 threads x
- * `iters` x 8 instructions.  op: 0 v_mad_u64_u32, 1 v_mad_u64_u32 +
- * v_addc_co_u32 (one multiply-accumulate of the kernels), 2 v_add_u32,
+ * `iters` x 8 instructions.  op: 0 v_mad_u64_u32 (8 independent chains),
+ * 1 v_mad_u64_u32 in one dependent chain (what one chain sustains), 2 v_add_u32,
  * 3 v_mul_lo_u32, 4 v_fma_f64.  Gives the denominator of the INT32
  * multiply-add roofline. */
 int pgx_valu_rate(int device, int op, uint32_t blocks, uint32_t iters, double* per_sec);

@@ -12,46 +12,31 @@
 namespace {
 
 template <int S>
-uint32_t one_candidate(const uint32_t* x, uint32_t limbs, uint32_t mode, const uint32_t* bases,
-                       uint32_t nbases, uint32_t* res_out) {
-  uint32_t n[S];
-  for (int j = 0; j < S; ++j) n[j] = (j < (int)limbs) ? x[j] : 0u;
+uint32_t one_candidate(const uint32_t* x, uint32_t limbs, uint32_t mode, const pgx::Bases& bases,
+                       uint32_t* res_out) {
+  constexpr int W = pgx::Radix<S>::W32;
+  uint32_t w[W], rw[W] = {};
+  for (int j = 0; j < W; ++j) w[j] = (j < (int)limbs) ? x[j] : 0u;
   uint32_t spill = 0;
-  for (uint32_t j = S; j < limbs; ++j) spill |= x[j];
-  const int L = pgx::bit_length<S>(n);
-  uint32_t r[S] = {};
+  for (uint32_t j = W; j < limbs; ++j) spill |= x[j];
   uint32_t v = PGX_VERDICT_INVALID;
-  if (spill == 0 && pgx::fits<S>(n, L)) {
-    const int t = pgx::ctz_nm1<S>(n);
-    const uint32_t ninv = pgx::neg_inv32(n[0]);
-    auto limb = [&](int j) { return x[j]; };
-    if (mode == PGX_MODE_FERMAT) {
-      v = pgx::test_base2<S>(n, ninv, L, t, limb, r);
-    } else {
-      const uint32_t need = (mode == PGX_MODE_STRONG) ? pgx::FLAG_SPRP : pgx::FLAG_QUIRK;
-      uint32_t passed = 0;
-      for (uint32_t b = 0; b < nbases; ++b) {
-        uint32_t scratch[S];
-        uint32_t(&dst)[S] = b == 0 ? r : scratch;
-        uint32_t f = (bases[b] == 2u) ? pgx::test_base2<S>(n, ninv, L, t, limb, dst)
-                                      : pgx::test_base<S>(bases[b], n, ninv, L, t, limb, dst);
-        if (!(f & need)) break;
-        ++passed;
-      }
-      v = passed;
-    }
+  if (spill == 0) {
+    auto word = [&](int j) { return x[j]; };
+    v = (mode == PGX_MODE_FERMAT) ? pgx::prp_candidate<S, true>(w, word, mode, bases, rw)
+                                  : pgx::prp_candidate<S, false>(w, word, mode, bases, rw);
   }
   if (res_out)
-    for (uint32_t j = 0; j < limbs; ++j) res_out[j] = (j < (uint32_t)S) ? r[j] : 0u;
+    for (uint32_t j = 0; j < limbs; ++j) res_out[j] = (j < (uint32_t)W) ? rw[j] : 0u;
   return v;
 }
 
-typedef uint32_t (*OneFn)(const uint32_t*, uint32_t, uint32_t, const uint32_t*, uint32_t,
-                          uint32_t*);
+typedef uint32_t (*OneFn)(const uint32_t*, uint32_t, uint32_t, const pgx::Bases&, uint32_t*);
+
+constexpr int kMaxS = 80;
 
 template <int... Ss>
 struct Table {
-  OneFn fn[PGX_MAX_KERNEL_LIMBS + 1] = {};
+  OneFn fn[kMaxS + 1] = {};
   Table() { ((fn[Ss] = &one_candidate<Ss>), ...); }
 };
 
@@ -65,28 +50,31 @@ const Table<PGX_EMU_S_VALUES> table;
 extern "C" {
 
 // Runs the kernel's per-candidate routine on the CPU with kernel limb count S
-// (0 = the smallest that fits max_bits).  Candidate-major limbs in, verdict
+// (radix limbs; 0 = the smallest built one that holds every candidate).  Candidate-major limbs in, verdict
 // words (and residues, candidate-major, if res != NULL) out.  Returns the S used,
 // or 0 if none is available.
 uint32_t pgx_lane_emu(const uint32_t* limbs_le, size_t n, uint32_t limbs, uint32_t S,
                       uint32_t mode, const uint32_t* bases, uint32_t nbases, uint32_t* verdict,
                       uint32_t* res) {
+  if (nbases > PGX_MAX_BASES) return 0;
+  pgx::Bases b{};
+  b.count = nbases;
+  for (uint32_t k = 0; k < nbases; ++k) b.a[k] = bases[k];
   if (S == 0) {
-    uint32_t max_bits = 0;
+    int max_bits = 0;
     for (size_t i = 0; i < n; ++i)
       for (int j = (int)limbs - 1; j >= 0; --j)
         if (limbs_le[i * limbs + j]) {
-          uint32_t b = 32u * j + 32u - __builtin_clz(limbs_le[i * limbs + j]);
-          if (b > max_bits) max_bits = b;
+          int bits = 32 * j + 32 - __builtin_clz(limbs_le[i * limbs + j]);
+          if (bits > max_bits) max_bits = bits;
           break;
         }
-    S = (max_bits + 2 + 31) / 32;
-    while (S <= PGX_MAX_KERNEL_LIMBS && !table.fn[S]) ++S;
+    for (S = 1; S <= (uint32_t)kMaxS; ++S)
+      if (table.fn[S] && (S <= 31 ? 29 : 28) * (int)S - 4 >= max_bits) break;
   }
-  if (S == 0 || S > PGX_MAX_KERNEL_LIMBS || !table.fn[S]) return 0;
+  if (S == 0 || S > (uint32_t)kMaxS || !table.fn[S]) return 0;
   for (size_t i = 0; i < n; ++i)
-    verdict[i] = table.fn[S](limbs_le + i * limbs, limbs, mode, bases, nbases,
-                             res ? res + i * limbs : nullptr);
+    verdict[i] = table.fn[S](limbs_le + i * limbs, limbs, mode, b, res ? res + i * limbs : nullptr);
   return S;
 }
 

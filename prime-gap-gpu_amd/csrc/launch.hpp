@@ -6,13 +6,9 @@
 #include <stdint.h>
 
 #include "../../include/pgx.h"
+#include "prp.hpp"
 
 namespace pgx {
-
-struct Bases {
-  uint32_t count;
-  uint32_t a[PGX_MAX_BASES];
-};
 
 // Arguments of one plain (one launch per batch) PRP launch.
 struct PrpArgs {
@@ -29,14 +25,23 @@ struct PrpArgs {
 
 typedef hipError_t (*PrpLaunchFn)(const PrpArgs&, hipStream_t);
 
-// Launchers indexed by kernel limb count S (1..PGX_MAX_KERNEL_LIMBS); an entry
-// is null when no translation unit instantiated that S.
+// Kernel limb counts: S radix-2^RB limbs hold candidates of up to
+// RB*S - 4 bits (RB = 29 up to 31 limbs, 28 above).
+constexpr int kMaxKernelLimbs = 80;
+inline int kernel_capacity_bits(int S) { return (S <= 31 ? 29 : 28) * S - 4; }
+
+// Launchers indexed by kernel limb count S (1..kMaxKernelLimbs); an entry is
+// null when no translation unit instantiated that S.
 PrpLaunchFn* prp_table();
+
+// Smallest instantiated S whose capacity holds max_bits, or 0.
+inline int kernel_for_bits(uint32_t max_bits) {
+  for (int S = 1; S <= kMaxKernelLimbs; ++S)
+    if (prp_table()[S] && kernel_capacity_bits(S) >= (int)max_bits) return S;
+  return 0;
+}
 
 // Threads per block of the plain kernel (4 wavefronts).
 constexpr uint32_t kBlock = 256;
-
-// Smallest kernel limb count for candidates of at most max_bits bits.
-inline uint32_t kernel_limbs_for_bits(uint32_t max_bits) { return (max_bits + 2 + 31) / 32; }
 
 }  // namespace pgx

@@ -1,22 +1,32 @@
 // Multi-limb Montgomery arithmetic for one candidate per lane (gfx950).
 //
-// Numbers are S little-endian 32-bit limbs held in VGPRs.  S is a compile-time
-// limb count chosen per bucket so that every candidate N of the bucket satisfies
-// N < 2^(32*S - 2), i.e. 4N < R with R = 2^(32*S).  That slack lets every value
-// stay in the "lazy" range [0, 2N) through the whole exponentiation: the
-// Montgomery product of two lazy values is again lazy, so no conditional
-// subtraction is needed inside the hot loop.
+// Registers hold numbers in radix 2^RB (RB = 29 for up to 31 limbs, 28 above),
+// S limbs, each limb < 2^RB.  Candidates arrive and residues leave as 32-bit
+// limbs (the reference's layout, miller_rabin.h:39-54, 142-145); the radix is
+// internal to the registers.
 //
-// The product is formed by finely-integrated product scanning: column k of the
-// square x^2 and column k of the reduction m*N are summed into one 96-bit
-// accumulator, the reduction digit m_k is derived from column k's low word, and
-// the column's low word is retired.  Each 32x32->64 product is one
-// v_mad_u64_u32 plus one carry add into the accumulator's third word.
+// Why not radix 2^32: v_mad_u64_u32 accumulates into 64 bits, and a 32x32-bit
+// product leaves no headroom, so every product would need a second instruction
+// to catch the carry (through an SGPR, with a hazard wait state).  With RB-bit
+// limbs a product is < 2^(2 RB) and a whole column of up to 2S+1 products
+// (cross products counted twice, the square, the reduction products) plus the
+// incoming carry stays below 2^64 (RB = 29: 2S+1 <= 63; RB = 28: 2S+1 <= 255).
+// Every product is then exactly one v_mad_u64_u32 into a 64-bit accumulator.
 //
-// The reference performs the same mathematical operation with CGBN's
-// cgbn_mont_sqr / cgbn_mont_mul across a TPI-thread group
-// (miller_rabin.h:181, 203, 209); here one lane owns a whole candidate, so no
-// cross-lane traffic is needed in the hot loop.
+// Montgomery: R = 2^(RB*S) with R >= 16 N.  Values are kept "lazy": after a
+// Montgomery square or product x < 2N; after a doubling x < 4N; and a
+// Montgomery square of any x < 4N returns < (16N^2 + RN)/R < 2N.  So the base-2
+// doubling is a bare shift by one bit and no conditional subtraction appears
+// in the hot loop.
+//
+// The squaring is finely-integrated product scanning: column k of x^2 (cross
+// products accumulated once and doubled) and of m*N share one accumulator, the
+// reduction digit m_k comes from column k's low RB bits, and the column's low
+// limb is retired.  Two accumulators per product kind keep independent mad
+// chains in flight.  The reference performs the same operation with CGBN's
+// cgbn_mont_sqr / cgbn_mont_mul across a TPI-thread group (miller_rabin.h:181,
+// 203, 209); here one lane owns a whole candidate, so the hot loop has no
+// cross-lane traffic.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,24 +36,21 @@
 
 namespace pgx {
 
-// acc(hi:lo) += a * b, with lo 64 bits and hi the third 32-bit word.
-// On the device this is v_mad_u64_u32 with its carry-out in an SGPR pair,
-// consumed by v_addc_co_u32 (two instructions per product; the portable
-// expression below compiles to four).  The host build of the same routine is
-// what the CPU tests exercise.
-PGX_FN void mac(uint64_t& lo, uint32_t& hi, uint32_t a, uint32_t b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  uint64_t c;
-  asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(lo), "=s"(c) : "v"(a), "v"(b));
-  asm("v_addc_co_u32 %0, %1, %0, 0, %1" : "+v"(hi), "+s"(c));
-#else
-  uint64_t p = (uint64_t)a * b;
-  lo += p;
-  hi += (lo < p) ? 1u : 0u;
-#endif
-}
+// Radix of a kernel with S limbs.
+template <int S>
+struct Radix {
+  static constexpr int RB = (S <= 31) ? 29 : 28;
+  static constexpr uint32_t MASK = (1u << RB) - 1u;
+  // largest bit length a candidate may have: R = 2^(RB*S) >= 16 N
+  static constexpr int MAX_BITS = RB * S - 4;
+  // 32-bit words spanned by S radix limbs
+  static constexpr int W32 = (RB * S + 31) / 32;
+};
 
-// -N^-1 mod 2^32 for odd n0 (Newton iteration, 3 -> 6 -> 12 -> 24 -> 48 bits).
+PGX_FN uint64_t mul64(uint32_t a, uint32_t b) { return (uint64_t)a * b; }
+
+// -N^-1 mod 2^32 for odd n0 (Newton iteration, 3 -> 6 -> 12 -> 24 -> 48 bits);
+// its low RB bits are -N^-1 mod 2^RB.
 PGX_FN uint32_t neg_inv32(uint32_t n0) {
   uint32_t inv = n0;
   inv *= 2u - n0 * inv;
@@ -53,155 +60,154 @@ PGX_FN uint32_t neg_inv32(uint32_t n0) {
   return 0u - inv;
 }
 
-// x <- x^2 / R mod N (lazy: x < 2N in, x < 2N out).
+// x <- x^2 / R mod N (lazy: x < 4N in, x < 2N out).
 template <int S>
 PGX_FN void mont_sqr(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv) {
+  constexpr int RB = Radix<S>::RB;
+  constexpr uint32_t MASK = Radix<S>::MASK;
   uint32_t m[S];
-  uint64_t t = 0;
-  uint32_t t2 = 0;
+  uint64_t carry = 0;
 #pragma unroll
   for (int k = 0; k < 2 * S - 1; ++k) {
     const int i0 = (k < S) ? 0 : k - S + 1;
-    // cross products x_i x_j, i < j, i + j = k
-    uint64_t c = 0;
-    uint32_t c2 = 0;
+    uint64_t c0 = 0, c1 = 0;
 #pragma unroll
-    for (int i = i0; i < k - i; ++i) mac(c, c2, x[i], x[k - i]);
-    c2 = (c2 << 1) | (uint32_t)(c >> 63);
-    c <<= 1;
-    if ((k & 1) == 0) mac(c, c2, x[k >> 1], x[k >> 1]);
-    t += c;
-    t2 += c2 + ((t < c) ? 1u : 0u);
-    // reduction products m_i n_{k-i}
+    for (int i = i0; i < k - i; ++i) {
+      if ((i - i0) & 1) c1 += mul64(x[i], x[k - i]);
+      else c0 += mul64(x[i], x[k - i]);
+    }
+    uint64_t t0 = carry, t1 = 0;
+    if ((k & 1) == 0) t1 += mul64(x[k >> 1], x[k >> 1]);
     const int i1 = (k < S) ? k : S;
 #pragma unroll
-    for (int i = i0; i < i1; ++i) mac(t, t2, m[i], n[k - i]);
-    if (k < S) {
-      m[k] = (uint32_t)t * ninv;
-      mac(t, t2, m[k], n[0]);
-    } else {
-      x[k - S] = (uint32_t)t;
+    for (int i = i0; i < i1; ++i) {
+      if ((i - i0) & 1) t1 += mul64(m[i], n[k - i]);
+      else t0 += mul64(m[i], n[k - i]);
     }
-    t = (t >> 32) | ((uint64_t)t2 << 32);
-    t2 = 0;
+    uint64_t t = t0 + t1 + ((c0 + c1) << 1);
+    if (k < S) {
+      m[k] = ((uint32_t)t * ninv) & MASK;
+      t += mul64(m[k], n[0]);  // low RB bits become zero
+    } else {
+      x[k - S] = (uint32_t)t & MASK;
+    }
+    carry = t >> RB;
   }
-  x[S - 1] = (uint32_t)t;
+  x[S - 1] = (uint32_t)carry;
 }
 
-// x <- x * y / R mod N (lazy).  x and y must not alias.
+// x <- x * y / R mod N (x < 4N, y < 4N with x*y < 16N^2; result < 2N).
+// x and y must not alias.
 template <int S>
 PGX_FN void mont_mul(uint32_t (&x)[S], const uint32_t (&y)[S], const uint32_t (&n)[S],
                      uint32_t ninv) {
+  constexpr int RB = Radix<S>::RB;
+  constexpr uint32_t MASK = Radix<S>::MASK;
   uint32_t m[S];
-  uint64_t t = 0;
-  uint32_t t2 = 0;
+  uint64_t carry = 0;
 #pragma unroll
   for (int k = 0; k < 2 * S - 1; ++k) {
     const int i0 = (k < S) ? 0 : k - S + 1;
     const int i1 = (k < S) ? k : S - 1;
+    uint64_t t0 = carry, t1 = 0;
 #pragma unroll
-    for (int i = i0; i <= i1; ++i) mac(t, t2, x[i], y[k - i]);
+    for (int i = i0; i <= i1; ++i) {
+      if ((i - i0) & 1) t1 += mul64(x[i], y[k - i]);
+      else t0 += mul64(x[i], y[k - i]);
+    }
     const int r1 = (k < S) ? k : S;
 #pragma unroll
-    for (int i = i0; i < r1; ++i) mac(t, t2, m[i], n[k - i]);
-    if (k < S) {
-      m[k] = (uint32_t)t * ninv;
-      mac(t, t2, m[k], n[0]);
-    } else {
-      x[k - S] = (uint32_t)t;
+    for (int i = i0; i < r1; ++i) {
+      if ((i - i0) & 1) t0 += mul64(m[i], n[k - i]);
+      else t1 += mul64(m[i], n[k - i]);
     }
-    t = (t >> 32) | ((uint64_t)t2 << 32);
-    t2 = 0;
+    uint64_t t = t0 + t1;
+    if (k < S) {
+      m[k] = ((uint32_t)t * ninv) & MASK;
+      t += mul64(m[k], n[0]);
+    } else {
+      x[k - S] = (uint32_t)t & MASK;
+    }
+    carry = t >> RB;
   }
-  x[S - 1] = (uint32_t)t;
+  x[S - 1] = (uint32_t)carry;
 }
 
-// x <- x / R mod N.  For x < 2N the result is < N + 1, hence canonical for any x
-// that is not a multiple of N.
+// x <- x / R mod N.  For x < 4N the result is < N + 1, hence canonical for
+// every x that is not a multiple of N.
 template <int S>
 PGX_FN void redc(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv) {
+  uint32_t one[S];
 #pragma unroll
-  for (int i = 0; i < S; ++i) {
-    uint32_t m = x[0] * ninv;
-    uint64_t c = ((uint64_t)m * n[0] + x[0]) >> 32;
-#pragma unroll
-    for (int j = 1; j < S; ++j) {
-      uint64_t v = (uint64_t)m * n[j] + x[j] + c;
-      x[j - 1] = (uint32_t)v;
-      c = v >> 32;
-    }
-    x[S - 1] = (uint32_t)c;
-  }
+  for (int j = 0; j < S; ++j) one[j] = (j == 0) ? 1u : 0u;
+  mont_mul<S>(x, one, n, ninv);
 }
 
-// d <- x - n; returns 1 when it borrowed (x < n).
+// Lazy doubling: when bit is 1, x <- 2x (a one-bit shift across limbs, no
+// reduction: x < 2N becomes < 4N).  bit must be 0 or 1.
+template <int S>
+PGX_FN void dbl_lazy(uint32_t (&x)[S], uint32_t bit) {
+  constexpr int RB = Radix<S>::RB;
+  constexpr uint32_t MASK = Radix<S>::MASK;
+#pragma unroll
+  for (int j = S - 1; j >= 1; --j)
+    x[j] = ((x[j] << bit) | (x[j - 1] >> (RB - bit))) & MASK;
+  x[0] = (x[0] << bit) & MASK;
+}
+
+// d <- x - n over normalized limbs; returns 1 when it borrowed (x < n).
 template <int S>
 PGX_FN uint32_t sub_borrow(uint32_t (&d)[S], const uint32_t (&x)[S], const uint32_t (&n)[S]) {
+  constexpr int RB = Radix<S>::RB;
+  constexpr uint32_t MASK = Radix<S>::MASK;
   uint32_t br = 0;
 #pragma unroll
   for (int j = 0; j < S; ++j) {
-    uint64_t v = (uint64_t)x[j] - n[j] - br;
-    d[j] = (uint32_t)v;
-    br = (uint32_t)(v >> 32) & 1u;
+    const uint32_t v = x[j] - n[j] - br;  // wraps when negative
+    d[j] = v & MASK;
+    br = v >> 31;  // |v| < 2^RB + 1 < 2^31, so bit 31 is the sign
   }
+  (void)RB;
   return br;
 }
 
-// Lazy doubling: when bit is set, x <- 2 * (x >= N ? x - N : x), keeping x < 2N.
+// x <- x - N when x >= N.
 template <int S>
-PGX_FN void dbl_lazy(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t bit) {
+PGX_FN void sub_if_ge(uint32_t (&x)[S], const uint32_t (&n)[S]) {
   uint32_t d[S];
   const uint32_t lt = sub_borrow<S>(d, x, n);
-  uint32_t prev = 0;
 #pragma unroll
-  for (int j = 0; j < S; ++j) {
-    uint32_t y = lt ? x[j] : d[j];
-    uint32_t z = (y << 1) | (prev >> 31);
-    prev = y;
-    x[j] = bit ? z : x[j];
-  }
+  for (int j = 0; j < S; ++j) x[j] = lt ? x[j] : d[j];
+}
+
+// Canonical form of a lazy value x < 4N.
+template <int S>
+PGX_FN void canon(uint32_t (&x)[S], const uint32_t (&n)[S]) {
+  sub_if_ge<S>(x, n);
+  sub_if_ge<S>(x, n);
+  sub_if_ge<S>(x, n);
 }
 
 // Full doubling for canonical v < N: v <- 2v mod N.
 template <int S>
 PGX_FN void dbl_full(uint32_t (&v)[S], const uint32_t (&n)[S]) {
-  uint32_t y[S];
-  uint32_t prev = 0;
-#pragma unroll
-  for (int j = 0; j < S; ++j) {
-    y[j] = (v[j] << 1) | (prev >> 31);
-    prev = v[j];
-  }
-  uint32_t d[S];
-  const uint32_t lt = sub_borrow<S>(d, y, n);
-#pragma unroll
-  for (int j = 0; j < S; ++j) v[j] = lt ? y[j] : d[j];
+  dbl_lazy<S>(v, 1u);
+  sub_if_ge<S>(v, n);
 }
 
 // v <- (v + w) mod N for canonical v, w < N.
 template <int S>
 PGX_FN void add_full(uint32_t (&v)[S], const uint32_t (&w)[S], const uint32_t (&n)[S]) {
-  uint32_t y[S];
+  constexpr int RB = Radix<S>::RB;
+  constexpr uint32_t MASK = Radix<S>::MASK;
   uint32_t c = 0;
 #pragma unroll
   for (int j = 0; j < S; ++j) {
-    uint64_t s = (uint64_t)v[j] + w[j] + c;
-    y[j] = (uint32_t)s;
-    c = (uint32_t)(s >> 32);
+    const uint32_t s = v[j] + w[j] + c;
+    v[j] = s & MASK;
+    c = s >> RB;
   }
-  uint32_t d[S];
-  const uint32_t lt = sub_borrow<S>(d, y, n);
-#pragma unroll
-  for (int j = 0; j < S; ++j) v[j] = lt ? y[j] : d[j];
-}
-
-// Canonical form of a lazy value: x < 2N -> x mod N.
-template <int S>
-PGX_FN void canon(uint32_t (&x)[S], const uint32_t (&n)[S]) {
-  uint32_t d[S];
-  const uint32_t lt = sub_borrow<S>(d, x, n);
-#pragma unroll
-  for (int j = 0; j < S; ++j) x[j] = lt ? x[j] : d[j];
+  sub_if_ge<S>(v, n);
 }
 
 template <int S>
@@ -215,51 +221,92 @@ PGX_FN bool equal(const uint32_t (&a)[S], const uint32_t (&b)[S]) {
 // a == N - b  <=>  a + b == N (both canonical).
 template <int S>
 PGX_FN bool is_neg(const uint32_t (&a)[S], const uint32_t (&b)[S], const uint32_t (&n)[S]) {
+  constexpr int RB = Radix<S>::RB;
+  constexpr uint32_t MASK = Radix<S>::MASK;
   uint32_t c = 0, diff = 0;
 #pragma unroll
   for (int j = 0; j < S; ++j) {
-    uint64_t s = (uint64_t)a[j] + b[j] + c;
-    diff |= (uint32_t)s ^ n[j];
-    c = (uint32_t)(s >> 32);
+    const uint32_t s = a[j] + b[j] + c;
+    diff |= (s & MASK) ^ n[j];
+    c = s >> RB;
   }
   return (diff | c) == 0;
 }
 
-// Bit length of an S-limb number (0 for zero).
+// v <- 2^bit (bit < RB*S).
 template <int S>
-PGX_FN int bit_length(const uint32_t (&n)[S]) {
+PGX_FN void set_pow2(uint32_t (&v)[S], int bit) {
+  constexpr int RB = Radix<S>::RB;
+  const int q = bit / RB, r = bit - q * RB;
+#pragma unroll
+  for (int j = 0; j < S; ++j) v[j] = (j == q) ? (1u << r) : 0u;
+}
+
+// one <- R mod N = 2^(RB*S) mod N, by doubling 2^(L-1) (which is < N).
+template <int S>
+PGX_FN void r_mod_n(uint32_t (&one)[S], const uint32_t (&n)[S], int L) {
+  constexpr int RB = Radix<S>::RB;
+  set_pow2<S>(one, L - 1);
+  const int steps = RB * S - (L - 1);
+  for (int c = 0; c < steps; ++c) dbl_full<S>(one, n);
+}
+
+// ---- conversion between 32-bit words and radix-2^RB limbs ----------------
+
+// Radix-RB limbs of a number given as W32 32-bit words.
+template <int S>
+PGX_FN void from_words(uint32_t (&x)[S], const uint32_t (&w)[Radix<S>::W32]) {
+  constexpr int RB = Radix<S>::RB;
+  constexpr int W = Radix<S>::W32;
+  constexpr uint32_t MASK = Radix<S>::MASK;
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int b = RB * j, q = b >> 5, r = b & 31;
+    uint32_t v = w[q] >> r;
+    if (r + RB > 32 && q + 1 < W) v |= w[q + 1] << (32 - r);
+    x[j] = v & MASK;
+  }
+}
+
+// 32-bit words of a number given as S radix-RB limbs.
+template <int S>
+PGX_FN void to_words(uint32_t (&w)[Radix<S>::W32], const uint32_t (&x)[S]) {
+  constexpr int RB = Radix<S>::RB;
+  constexpr int W = Radix<S>::W32;
+#pragma unroll
+  for (int q = 0; q < W; ++q) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const int lo = RB * j;  // limb j covers bits [lo, lo + RB)
+      if (lo + RB <= 32 * q || lo >= 32 * q + 32) continue;
+      const int sh = lo - 32 * q;
+      v |= (sh >= 0) ? (x[j] << sh) : (x[j] >> -sh);
+    }
+    w[q] = v;
+  }
+}
+
+// Bit length of a number given as W 32-bit words (0 for zero).
+template <int W>
+PGX_FN int bit_length_words(const uint32_t (&w)[W]) {
   int L = 0;
 #pragma unroll
-  for (int j = 0; j < S; ++j)
-    if (n[j]) L = 32 * j + 32 - __builtin_clz(n[j]);
+  for (int j = 0; j < W; ++j)
+    if (w[j]) L = 32 * j + 32 - __builtin_clz(w[j]);
   return L;
 }
 
-// Trailing zero count of N - 1 for odd N > 1.
-template <int S>
-PGX_FN int ctz_nm1(const uint32_t (&n)[S]) {
+// Trailing zero count of N - 1 for odd N > 1, N given as W 32-bit words.
+template <int W>
+PGX_FN int ctz_nm1_words(const uint32_t (&w)[W]) {
   int t = -1;
 #pragma unroll
-  for (int j = S - 1; j >= 0; --j) {
-    uint32_t w = (j == 0) ? (n[0] & ~1u) : n[j];
-    if (w) t = 32 * j + __builtin_ctz(w);
+  for (int j = W - 1; j >= 0; --j) {
+    const uint32_t v = (j == 0) ? (w[0] & ~1u) : w[j];
+    if (v) t = 32 * j + __builtin_ctz(v);
   }
   return t;
-}
-
-// v <- 2^(L-1), the largest power of two below an L-bit odd N.
-template <int S>
-PGX_FN void set_pow2(uint32_t (&v)[S], int bit) {
-#pragma unroll
-  for (int j = 0; j < S; ++j) v[j] = (j == (bit >> 5)) ? (1u << (bit & 31)) : 0u;
-}
-
-// one <- R mod N = 2^(32S) mod N, by doubling 2^(L-1) (which is < N).
-template <int S>
-PGX_FN void r_mod_n(uint32_t (&one)[S], const uint32_t (&n)[S], int L) {
-  set_pow2<S>(one, L - 1);
-  const int steps = 32 * S - (L - 1);
-  for (int c = 0; c < steps; ++c) dbl_full<S>(one, n);
 }
 
 }  // namespace pgx

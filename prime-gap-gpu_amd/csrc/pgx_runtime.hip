@@ -15,7 +15,7 @@
 namespace pgx {
 
 PrpLaunchFn* prp_table() {
-  static PrpLaunchFn table[PGX_MAX_KERNEL_LIMBS + 1] = {};
+  static PrpLaunchFn table[kMaxKernelLimbs + 1] = {};
   return table;
 }
 
@@ -100,12 +100,10 @@ void release(pgx_runner* h) {
 int launch(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs, size_t n,
            uint32_t max_bits, uint32_t flags, uint32_t* d_verdict, uint32_t* d_res,
            uint32_t res_ld, hipStream_t stream) {
-  // smallest instantiated kernel limb count that holds max_bits + 2 bits
-  uint32_t S = pgx::kernel_limbs_for_bits(max_bits);
-  while (S <= PGX_MAX_KERNEL_LIMBS && !pgx::prp_table()[S]) ++S;
-  if (S > PGX_MAX_KERNEL_LIMBS)
-    return fail(h, PGX_E_INPUT, "candidates of %u bits exceed the widest kernel (%d limbs)",
-                max_bits, PGX_MAX_KERNEL_LIMBS);
+  const int S = pgx::kernel_for_bits(max_bits);
+  if (S == 0)
+    return fail(h, PGX_E_INPUT, "candidates of %u bits exceed the widest kernel (%d bits)",
+                max_bits, PGX_MAX_BITS);
   pgx::PrpLaunchFn fn = pgx::prp_table()[S];
   pgx::PrpArgs a{};
   a.cand = d_cand;
@@ -205,7 +203,7 @@ int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t fla
     const uint32_t L = bit_length(x, S);
     if (!(x[0] & 1u) || L < 2)
       return fail(h, PGX_E_INPUT, "candidate %zu is even or <= 1", i);
-    if (L + 2 > 32u * PGX_MAX_KERNEL_LIMBS)
+    if (L > PGX_MAX_BITS)
       return fail(h, PGX_E_INPUT, "candidate %zu has %u bits, more than the kernels support", i, L);
     max_bits = std::max(max_bits, L);
     for (uint32_t j = 0; j < S; ++j) h->h_cand[(size_t)j * n + i] = x[j];

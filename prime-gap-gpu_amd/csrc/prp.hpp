@@ -7,33 +7,40 @@
 //   QUIRK : SPRP  or  (t >= 2 and y_t == N-1)      (the reference squares t times,
 //           miller_rabin.h:236-244, so it also accepts a^(N-1) == -1)
 //   FERMAT: y_t == 1                                (a^(N-1) mod N == 1)
+// (a^(N-1) == -1 mod N has no odd solution N > 1, so QUIRK == SPRP; both are
+// computed anyway and the tests hold them equal.)
 // The exponentiation is left-to-right binary over the bits of N-1 (the
 // reference uses a fixed 2^W window table, miller_rabin.h:160-214; the result
-// does not depend on the chunking).  For base 2 the per-bit multiply is a lazy
-// doubling, O(S) instead of a second Montgomery product, so no table is needed.
+// does not depend on the chunking).  For base 2 the per-bit multiply is a
+// one-bit shift (lazy doubling), so base 2 needs no table and no second
+// Montgomery product.
 //
 // One loop covers all bits L-2 .. 0 of N-1: the bits below t are zero, so the
-// strong-test checks of y_0..y_{t-1} ride on the same squaring call site.  Only
-// one copy of the S^2 squaring body is emitted per base kind, which keeps the
-// loop inside the instruction cache.
+// strong-test checks of y_0..y_{t-1} ride on the same squaring call site and
+// only one copy of the unrolled squaring is emitted per base kind.
 #pragma once
 
+#include "../../include/pgx.h"
 #include "mont.hpp"
 
 namespace pgx {
 
 enum : uint32_t {
-  FLAG_FERMAT = 1u,  // a^(N-1) == 1
-  FLAG_SPRP = 2u,    // textbook strong probable prime to base a
-  FLAG_QUIRK = 4u,   // the reference's verdict (SPRP or t >= 2 and a^(N-1) == -1)
+  FLAG_FERMAT = PGX_FLAG_FERMAT,  // a^(N-1) == 1
+  FLAG_SPRP = PGX_FLAG_SPRP,      // textbook strong probable prime to base a
+  FLAG_QUIRK = PGX_FLAG_QUIRK,    // the reference's verdict
 };
 
-// Step that multiplies the running power by the base when an exponent bit is
-// set.  Base 2: lazy doubling.
+struct Bases {
+  uint32_t count;
+  uint32_t a[PGX_MAX_BASES];
+};
+
+// Base 2: multiply by the base = lazy doubling.
 template <int S>
 struct MulBy2 {
-  PGX_FN void operator()(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t, uint32_t bit) const {
-    dbl_lazy<S>(x, n, bit);
+  PGX_FN void operator()(uint32_t (&x)[S], const uint32_t (&)[S], uint32_t, uint32_t bit) const {
+    dbl_lazy<S>(x, bit);
   }
 };
 
@@ -47,11 +54,13 @@ struct MulByMont {
   }
 };
 
-// Shared exponentiation + strong-test tail.  x enters as Mont(a) (the leading
-// bit of N-1).  res receives a^(N-1) mod N (canonical); returns FLAG_* bits.
-template <int S, class LimbFn, class Step>
-PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv, int L,
-                             int t, LimbFn limb, const Step& step, uint32_t (&res)[S]) {
+// Exponentiation + strong-test tail.  x enters as Mont(a) (the leading bit of
+// N-1).  `word(j)` returns 32-bit word j of N; it is read once per 32 exponent
+// bits so the exponent never needs dynamically indexed registers.  res gets
+// a^(N-1) mod N (canonical); returns FLAG_* bits.
+template <int S, class WordFn, class Step>
+PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv, int L, int t,
+                             WordFn word, const Step& step, uint32_t (&res)[S]) {
   uint32_t one[S];
   bool sprp = false;
   if (t == L - 1) {  // N - 1 = 2^t: y_0 = a
@@ -59,12 +68,12 @@ PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t 
     canon<S>(x, n);
     sprp = equal<S>(x, one) || is_neg<S>(x, one, n);
   }
-  uint32_t word = 0;
+  uint32_t w = 0;
   for (int p = L - 2; p >= 0; --p) {
-    if (p == L - 2 || (p & 31) == 31) word = limb(p >> 5);
+    if (p == L - 2 || (p & 31) == 31) w = word(p >> 5);
     mont_sqr<S>(x, n, ninv);
     // bit p of N-1: bit p of N for p >= 1, zero for p == 0
-    step(x, n, ninv, (p > 0) ? ((word >> (p & 31)) & 1u) : 0u);
+    step(x, n, ninv, (p > 0) ? ((w >> (p & 31)) & 1u) : 0u);
     if (p <= t && p >= 1) {  // x = Mont(y_{t-p})
       if (p == t) r_mod_n<S>(one, n, L);
       canon<S>(x, n);
@@ -87,21 +96,19 @@ PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t 
   return f;
 }
 
-// Base-2 test.  `limb(j)` returns limb j of N (runtime j); it is read once per
-// 32 exponent bits so the exponent never needs dynamically indexed registers.
-template <int S, class LimbFn>
-PGX_FN uint32_t test_base2(const uint32_t (&n)[S], uint32_t ninv, int L, int t, LimbFn limb,
+template <int S, class WordFn>
+PGX_FN uint32_t test_base2(const uint32_t (&n)[S], uint32_t ninv, int L, int t, WordFn word,
                            uint32_t (&res)[S]) {
   uint32_t x[S];
   r_mod_n<S>(x, n, L);
   dbl_full<S>(x, n);  // Mont(2)
-  return powm_verdict<S>(x, n, ninv, L, t, limb, MulBy2<S>{}, res);
+  return powm_verdict<S>(x, n, ninv, L, t, word, MulBy2<S>{}, res);
 }
 
 // General base a >= 2 (the extra Miller-Rabin bases).
-template <int S, class LimbFn>
+template <int S, class WordFn>
 PGX_FN uint32_t test_base(uint32_t a, const uint32_t (&n)[S], uint32_t ninv, int L, int t,
-                          LimbFn limb, uint32_t (&res)[S]) {
+                          WordFn word, uint32_t (&res)[S]) {
   uint32_t am[S];
   {
     uint32_t one[S];
@@ -117,13 +124,48 @@ PGX_FN uint32_t test_base(uint32_t a, const uint32_t (&n)[S], uint32_t ninv, int
   uint32_t x[S];
 #pragma unroll
   for (int j = 0; j < S; ++j) x[j] = am[j];
-  return powm_verdict<S>(x, n, ninv, L, t, limb, MulByMont<S>{am}, res);
+  return powm_verdict<S>(x, n, ninv, L, t, word, MulByMont<S>{am}, res);
 }
 
-// Validity of a candidate for a kernel of S limbs: odd, > 1 and 4N < 2^(32S).
-template <int S>
-PGX_FN bool fits(const uint32_t (&n)[S], int L) {
-  return (n[0] & 1u) && L >= 2 && L <= 32 * S - 2;
+// Whole test of one candidate given as W32 words (w).  Returns the verdict word
+// of `mode` (PGX_VERDICT_INVALID if N is even, <= 1 or wider than the kernel)
+// and writes the first base's a^(N-1) mod N as W32 words into res_w.
+template <int S, bool kBase2Only, class WordFn>
+PGX_FN uint32_t prp_candidate(const uint32_t (&w)[Radix<S>::W32], WordFn word, uint32_t mode,
+                              const Bases& bases, uint32_t (&res_w)[Radix<S>::W32]) {
+  constexpr int W = Radix<S>::W32;
+  const int L = bit_length_words<W>(w);
+  uint32_t r[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) r[j] = 0;
+  uint32_t v = PGX_VERDICT_INVALID;
+  if ((w[0] & 1u) && L >= 2 && L <= Radix<S>::MAX_BITS) {
+    const int t = ctz_nm1_words<W>(w);
+    uint32_t n[S];
+    from_words<S>(n, w);
+    const uint32_t ninv = neg_inv32(w[0]);
+    if (kBase2Only) {
+      v = test_base2<S>(n, ninv, L, t, word, r);
+    } else {
+      const uint32_t need = (mode == PGX_MODE_STRONG) ? FLAG_SPRP : FLAG_QUIRK;
+      uint32_t passed = 0;
+      for (uint32_t b = 0; b < bases.count; ++b) {
+        uint32_t f;
+        if (b == 0) {
+          f = (bases.a[0] == 2u) ? test_base2<S>(n, ninv, L, t, word, r)
+                                 : test_base<S>(bases.a[0], n, ninv, L, t, word, r);
+        } else {
+          uint32_t scratch[S];
+          f = test_base<S>(bases.a[b], n, ninv, L, t, word, scratch);
+        }
+        if (!(f & need)) break;
+        ++passed;
+      }
+      v = passed;
+    }
+  }
+  to_words<S>(res_w, r);
+  return v;
 }
 
 }  // namespace pgx

@@ -20,52 +20,32 @@ namespace pgx {
 
 template <int S, bool kBase2Only>
 __global__ __launch_bounds__(kBlock) void k_prp_plain(PrpArgs a) {
+  constexpr int W = Radix<S>::W32;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const uint32_t* c = a.cand + i;
   const size_t ld = a.ld;
-  uint32_t n[S];
+  uint32_t w[W];
 #pragma unroll
-  for (int j = 0; j < S; ++j) n[j] = (j < (int)a.buf_limbs) ? c[j * ld] : 0u;
-  uint32_t spill = 0;
-  for (uint32_t j = S; j < a.buf_limbs; ++j) spill |= c[j * ld];
-  const int L = bit_length<S>(n);
+  for (int j = 0; j < W; ++j) w[j] = (j < (int)a.buf_limbs) ? c[j * ld] : 0u;
+  uint32_t spill = 0;  // words above the kernel's width must be zero
+  for (uint32_t j = W; j < a.buf_limbs; ++j) spill |= c[j * ld];
+  uint32_t rw[W];
   uint32_t v = PGX_VERDICT_INVALID;
-  uint32_t r[S];
+  if (spill == 0) {
+    v = prp_candidate<S, kBase2Only>(w, [&](int j) { return c[j * ld]; }, a.mode, a.bases, rw);
+  } else {
 #pragma unroll
-  for (int j = 0; j < S; ++j) r[j] = 0;
-  if (spill == 0 && fits<S>(n, L)) {
-    const int t = ctz_nm1<S>(n);
-    const uint32_t ninv = neg_inv32(n[0]);
-    auto limb = [&](int j) { return c[j * ld]; };
-    if (kBase2Only) {
-      v = test_base2<S>(n, ninv, L, t, limb, r);
-    } else {
-      const uint32_t need = (a.mode == PGX_MODE_STRONG) ? FLAG_SPRP : FLAG_QUIRK;
-      uint32_t passed = 0;
-      for (uint32_t b = 0; b < a.bases.count; ++b) {
-        uint32_t f;
-        if (b == 0) {
-          f = (a.bases.a[0] == 2u) ? test_base2<S>(n, ninv, L, t, limb, r)
-                                   : test_base<S>(a.bases.a[0], n, ninv, L, t, limb, r);
-        } else {
-          uint32_t scratch[S];
-          f = test_base<S>(a.bases.a[b], n, ninv, L, t, limb, scratch);
-        }
-        if (!(f & need)) break;
-        ++passed;
-      }
-      v = passed;
-    }
+    for (int j = 0; j < W; ++j) rw[j] = 0;
   }
   a.verdict[i] = v;
   if (a.residue) {
     uint32_t* o = a.residue + i;
     const size_t rld = a.res_ld;
 #pragma unroll
-    for (int j = 0; j < S; ++j)
-      if (j < (int)a.buf_limbs) o[j * rld] = r[j];
-    for (uint32_t j = S; j < a.buf_limbs; ++j) o[j * rld] = 0u;
+    for (int j = 0; j < W; ++j)
+      if (j < (int)a.buf_limbs) o[j * rld] = rw[j];
+    for (uint32_t j = W; j < a.buf_limbs; ++j) o[j * rld] = 0u;
   }
 }
 

@@ -33,8 +33,8 @@ __global__ __launch_bounds__(256) void k_rate(uint32_t iters, uint32_t seed, uin
       if constexpr (OP == 0) {  // v_mad_u64_u32 alone (carry-out discarded)
         uint64_t c;
         asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(acc[k]), "=s"(c) : "v"(a), "v"(b));
-      } else if constexpr (OP == 1) {  // the kernels' multiply-accumulate pair
-        pgx::mac(acc[k], hi[k], a, b);
+      } else if constexpr (OP == 1) {  // one dependent multiply-accumulate chain per lane
+        acc[0] += pgx::mul64(a + (uint32_t)acc[0], b);
       } else if constexpr (OP == 2) {  // full-rate integer add
         asm volatile("v_add_u32 %0, %1, %0" : "+v"(hi[k]) : "v"(a));
       } else if constexpr (OP == 3) {  // v_mul_lo_u32
@@ -62,7 +62,8 @@ hipError_t launch(uint32_t blocks, uint32_t iters, uint32_t* out, hipStream_t s)
 extern "C" {
 
 /* Lane-instructions per second of instruction kind `op` on `device`:
- * 0 v_mad_u64_u32, 1 v_mad_u64_u32 + v_addc_co_u32 (one multiply-accumulate),
+ * 0 v_mad_u64_u32 (8 independent chains per lane: issue rate), 1 v_mad_u64_u32
+ * in one dependent chain per lane (latency bound at low occupancy),
  * 2 v_add_u32, 3 v_mul_lo_u32, 4 v_fma_f64.  `blocks` of 256 threads run
  * `iters` x 8 instructions per lane.  Result in *per_sec (for op 1: MACs/s). */
 int pgx_valu_rate(int device, int op, uint32_t blocks, uint32_t iters, double* per_sec) {

@@ -18,7 +18,7 @@
 //    the device.
 //  * gpu_state_and_checks no longer asks for a recompile per BITS
 //    (gpu_testing.cu:369-385): the kernel limb count is chosen per batch from
-//    the candidates' bit length, up to PGX_MAX_KERNEL_LIMBS limbs.
+//    the candidates' bit length, up to PGX_MAX_BITS bits.
 #include <gmp.h>
 #include <pthread.h>
 #include <unistd.h>
@@ -230,9 +230,8 @@ void gpu_state_and_checks(const mpz_t& K_in, const uint64_t m_end) {
     printf("\nERROR: m*K has up to %zu bits but GPU_BITS=%u\n\n", n_bits, kBits);
     exit(1);
   }
-  if (n_bits + 2 > 32u * PGX_MAX_KERNEL_LIMBS) {
-    printf("\nERROR: m*K has up to %zu bits, the kernels hold %u\n\n", n_bits,
-           32u * PGX_MAX_KERNEL_LIMBS - 2);
+  if (n_bits > PGX_MAX_BITS) {
+    printf("\nERROR: m*K has up to %zu bits, the kernels hold %d\n\n", n_bits, PGX_MAX_BITS);
     exit(1);
   }
 }

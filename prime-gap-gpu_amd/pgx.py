@@ -53,7 +53,7 @@ FLAG_FERMAT = 1
 FLAG_SPRP = 2
 FLAG_QUIRK = 4
 VERDICT_INVALID = 0xFFFFFFFF
-MAX_KERNEL_LIMBS = 64
+MAX_BITS = 2068
 
 EXPORTED = (
     "pgx_runner_create",
@@ -179,7 +179,7 @@ def gen_sieved(P: int, d: int, m: np.ndarray, x_max: int, per_m: int, limbs: int
     return out[:n], om[:n], ox[:n]
 
 
-VALU_OPS = {0: "v_mad_u64_u32", 1: "mad+addc", 2: "v_add_u32", 3: "v_mul_lo_u32",
+VALU_OPS = {0: "v_mad_u64_u32", 1: "v_mad_u64_u32_dependent", 2: "v_add_u32", 3: "v_mul_lo_u32",
             4: "v_fma_f64"}
 
 

@@ -117,12 +117,12 @@ def test_lane_routine_fixture_records(emu, vectors):
     recs = [r for r in vectors["kat"] + vectors["widths"] if int(r["n"], 16).bit_length() <= 734]
     ns = [int(r["n"], 16) for r in recs]
     L = pgx.to_limbs(ns, 24)
-    v, r = emu(L, 23, pgx.MODE_FERMAT)
+    v, r = emu(L, 26, pgx.MODE_FERMAT)  # 26 radix-2^29 limbs hold 750 bits
     assert pgx.from_limbs(r) == [int(x["res2"], 16) for x in recs]
     for i, rec in enumerate(recs):
         assert bool(v[i] & 1) == rec["fermat2"] and bool(v[i] & 2) == rec["sprp"]["2"]
     for mode, key in ((pgx.MODE_REFERENCE, "ref"), (pgx.MODE_STRONG, "sprp")):
-        v, _ = emu(L, 23, mode, (2, 3, 5, 7))
+        v, _ = emu(L, 26, mode, (2, 3, 5, 7))
         for i, rec in enumerate(recs):
             want = 0
             for a in "2357":
@@ -135,17 +135,18 @@ def test_lane_routine_fixture_records(emu, vectors):
 def test_lane_routine_sieved_vs_oracle(emu):
     m = pgx.gen_multipliers(0x5EED0001, 10**7, 10**7 + 10**4, 2310, 2)
     L, _, _ = pgx.gen_sieved(337, 2310, m, 200000, 200, 16)
-    for S in (15, 16):
+    for S in (17, 20):  # 462-bit candidates: 17 limbs is the narrowest that holds them
         v, r = emu(L, S, pgx.MODE_FERMAT)
         ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
         assert np.array_equal(v, ov) and np.array_equal(r, orr)
-    v, _ = emu(L, 15, pgx.MODE_REFERENCE, (2, 3, 5))
+    v, _ = emu(L, 17, pgx.MODE_REFERENCE, (2, 3, 5))
     assert np.array_equal(v, O.verdicts(L, O.MODE_REFERENCE, (2, 3, 5)))
 
 
 def test_lane_routine_rejects_what_does_not_fit(emu):
-    # even, one, and a value too wide for S=1 (needs 4N < 2^32)
-    L = pgx.to_limbs([10, 1, (1 << 31) - 1, (1 << 30) - 1], 1)
+    # even, one, and a value too wide for one radix-2^29 limb (R = 2^29 >= 16N
+    # allows 25-bit N)
+    L = pgx.to_limbs([10, 1, (1 << 26) - 1, (1 << 25) - 1], 1)
     v, _ = emu(L, 1, pgx.MODE_FERMAT)
     assert list(v[:3]) == [pgx.VERDICT_INVALID] * 3
     assert v[3] != pgx.VERDICT_INVALID
