@@ -56,6 +56,149 @@ def kernel_limbs(max_bits: int) -> int:
     return S
 
 
+def gen(P: int, d: int, words: int, n_m: int, per_m: int, seed: int) -> np.ndarray:
+    import pgx
+    m = pgx.gen_multipliers(seed, 10**7, 10**7 + 10**4, d, n_m)
+    L, _, _ = pgx.gen_sieved(P, d, m, X_MAX, per_m, words)
+    return L
+
+
+def median_bits(L: np.ndarray) -> int:
+    import pgx
+    return int(np.median([int(x).bit_length() for x in pgx.from_limbs(L[:1024])]))
+
+
+def time_ms(stream, fn, reps: int) -> float:
+    """Average ms of fn() over reps, HIP events on `stream`."""
+    import torch
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+MIXED = [(211, 16), (337, 16), (503, 24), (701, 32)]  # (P, buffer words), d = 210
+
+
+def bench_mixed(stream, peak: float, reps: int) -> dict:
+    """2^21 candidates, P in {211, 337, 503, 701}, d=210: one hipGraph launch
+    over four autotuned persistent buckets vs the plain kernel (one launch per
+    65536) on the same HBM-resident data."""
+    import torch
+
+    import oracle
+    import pgx
+    t0 = time.time()
+    data = [gen(P, 210, w, 32, 16384, SEED + P) for P, w in MIXED]
+    batches = [pgx.DeviceBatch(L) for L in data]
+    log(f"mixed: generated {sum(len(L) for L in data)} candidates in {time.time() - t0:.1f}s")
+    runner = pgx.Runner(1, limbs=16)
+    tunings, per_bucket = [], {}
+    for (P, w), b, L in zip(MIXED, batches, data):
+        t, rate = runner.autotune(b.bucket(), pgx.MODE_FERMAT)
+        tunings.append(t)
+        per_bucket[f"P{P}"] = {"bits": median_bits(L), "kernel_limbs": kernel_limbs(b.max_bits),
+                               "tuned": t.as_dict(), "tests_per_s": rate}
+        log(f"mixed: P={P} tuned {t.as_dict()} {rate / 1e6:.2f} M/s")
+    plan = pgx.Plan(runner, [b.bucket() for b in batches], pgx.MODE_FERMAT, residues=True,
+                    tunings=tunings)
+    plan.launch(stream.cuda_stream)
+    ms_graph = time_ms(stream, lambda: plan.launch(stream.cuda_stream), reps)
+    ok = True
+    for b, L in zip(batches, data):
+        ov, ores = oracle.verdicts(L[:1024], oracle.MODE_FERMAT, residues=True)
+        ok &= bool(np.array_equal(b.verdicts()[:1024], ov) and
+                   np.array_equal(b.residues()[:1024], ores))
+
+    def plain():
+        for b in batches:
+            for c0 in range(0, b.n, BATCH):
+                k = min(BATCH, b.n - c0)
+                runner.test_device(b.cand.data_ptr() + 4 * c0, b.ld, b.limbs, k, b.max_bits,
+                                   pgx.MODE_FERMAT, b.verdict.data_ptr() + 4 * c0,
+                                   b.res.data_ptr() + 4 * c0, b.ld, stream.cuda_stream)
+    plain()
+    ms_plain = time_ms(stream, plain, max(1, reps // 2))
+    total = sum(len(L) for L in data)
+    macs = sum(work_per_test(median_bits(L)) * len(L) for L in data)
+    plan.close()
+    runner.close()
+    return {
+        "workload": "P in {211,337,503,701}, d=210, 2^21 sieved candidates in 16/16/24/32-word "
+                    "buffers, 4 buckets by kernel limb count, one hipGraph launch",
+        "candidates": total,
+        "graph_ms": ms_graph, "graph_tests_per_s": total / (ms_graph * 1e-3),
+        "plain_ms": ms_plain, "plain_tests_per_s": total / (ms_plain * 1e-3),
+        "speedup_vs_plain": ms_plain / ms_graph,
+        "valu_frac": macs / (ms_graph * 1e-3) / peak,
+        "buckets": per_bucket,
+        "parity_sample_ok": ok,
+    }
+
+
+def bench_large(stream, peak: float, reps: int) -> dict:
+    """P=907, d=30030 (~1260-bit N in 64-word buffers), 2^18 candidates:
+    base-2 Fermat (persistent, autotuned), on-device compaction of the PRPs,
+    strong tests to bases 3 and 5 on the survivors only."""
+    import torch
+
+    import oracle
+    import pgx
+    L = gen(907, 30030, 64, 32, 8192, SEED + 907)
+    n = len(L)
+    a = pgx.DeviceBatch(L)
+    mr_verdict = torch.zeros(n, dtype=torch.int32, device="cuda")
+    mr_bucket = pgx.Bucket(a.cand.data_ptr(), a.ld, a.limbs, n, a.max_bits,
+                           mr_verdict.data_ptr(), 0, a.ld)
+    index = torch.zeros(n, dtype=torch.int32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int32, device="cuda")
+    fermat = pgx.Runner(1, limbs=64)
+    extra = pgx.Runner(1, limbs=64, bases=[3, 5])
+    t, rate = fermat.autotune(a.bucket(), pgx.MODE_FERMAT)
+    log(f"large: tuned {t.as_dict()} {rate / 1e6:.3f} M/s")
+
+    def fermat_only():
+        fermat.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True,
+                               stream=stream.cuda_stream)
+
+    def pipeline():
+        fermat_only()
+        fermat.compact(a.verdict.data_ptr(), n, pgx.FLAG_FERMAT, index.data_ptr(),
+                       count.data_ptr(), stream.cuda_stream)
+        extra.test_indexed(mr_bucket, index.data_ptr(), count.data_ptr(), pgx.MODE_STRONG,
+                           stream=stream.cuda_stream)
+    pipeline()
+    ms_fermat = time_ms(stream, fermat_only, reps)
+    ms_all = time_ms(stream, pipeline, reps)
+    k = int(count.item())
+    v = a.verdicts()
+    surv = np.sort(index.cpu().numpy()[:k])
+    mv = mr_verdict.cpu().numpy().view(np.uint32)
+    ov = oracle.verdicts(L[:1024], oracle.MODE_FERMAT)
+    s_sample = surv[:256]
+    ok = bool(np.array_equal(v[:1024], ov) and
+              np.array_equal(mv[s_sample], oracle.verdicts(L[s_sample], oracle.MODE_STRONG, (3, 5)))
+              and k == int((v & 1).sum()))
+    bits = median_bits(L)
+    fermat.close()
+    extra.close()
+    return {
+        "workload": "P=907 d=30030, 2^18 sieved candidates (~1260 bits) in 64-word buffers: "
+                    "Fermat base 2 -> device compaction -> strong bases 3,5 on survivors",
+        "candidates": n, "bits": bits, "kernel_limbs": kernel_limbs(a.max_bits),
+        "tuned": t.as_dict(),
+        "fermat_ms": ms_fermat, "fermat_tests_per_s": n / (ms_fermat * 1e-3),
+        "pipeline_ms": ms_all, "pipeline_tests_per_s": n / (ms_all * 1e-3),
+        "survivors": k,
+        "fermat_valu_frac": work_per_test(bits) * n / (ms_fermat * 1e-3) / peak,
+        "parity_sample_ok": ok,
+    }
+
+
 def cpu_baseline(cands: np.ndarray, target_cpu_s: float = 15.0) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the GMP path, timed as the reported CPU baseline
@@ -85,6 +228,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline plain-kernel workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path "
@@ -166,6 +311,11 @@ def main() -> int:
     peak = rate["v_mad_u64_u32"]
     log("VALU rates (lane-ops/s): " + ", ".join(f"{k}={v:.3e}" for k, v in rate.items() if v))
 
+    extra = {}
+    if not args.no_extras:
+        extra["mixed_graph"] = bench_mixed(stream, peak, max(2, args.steps // 4))
+        extra["large_fermat_mr"] = bench_large(stream, peak, max(2, args.steps // 4))
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cands)
@@ -207,6 +357,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "parity_sample_ok": parity,
+        "configs": extra,
     }
     print(json.dumps(line), flush=True)
     runner.close()

@@ -110,6 +110,68 @@ int pgx_test_device(pgx_runner_t* h, const uint32_t* d_cand, uint32_t ld, uint32
  * h is NULL).  Never NULL. */
 const char* pgx_last_error(const pgx_runner_t* h);
 
+/* ---- batched persistent launches ----------------------------------------
+ * The reference tests one fixed-BITS batch per launch from three host threads
+ * (gpu_testing.cu:203-354, miller_rabin.h:383-402) and needs one binary per
+ * BITS (gpu_testing.cu:369-385).  Here candidates of any mix of sizes are
+ * grouped into buckets (one kernel limb count each), every bucket runs as a
+ * persistent kernel whose resident wavefronts take chunks of candidates from
+ * a device counter, and a plan captures all buckets (parallel graph branches,
+ * counters reset by a fill-kernel node) into one hipGraph launch. */
+typedef struct {
+  const uint32_t* d_cand; /* limb-interleaved candidates: limb j of column p at d_cand[j*ld + p] */
+  uint32_t ld;            /* leading dimension, >= n */
+  uint32_t limbs;         /* rows stored in d_cand */
+  uint32_t n;             /* candidates (columns 0..n-1) */
+  uint32_t max_bits;      /* bound on every candidate's bit length (selects the kernel) */
+  uint32_t* d_verdict;    /* n verdict words (indexed by column) */
+  uint32_t* d_residues;   /* limbs rows of res_ld words, or NULL */
+  uint32_t res_ld;
+} pgx_bucket_t;
+
+typedef struct {
+  uint32_t blocks_per_cu;    /* resident workgroups per compute unit */
+  uint32_t waves_per_block;  /* wavefronts per workgroup, 1..4 */
+  uint32_t chunk;            /* candidates a wavefront takes per counter bump, multiple of 64 */
+  uint32_t lanes_per_number; /* lanes that share one candidate; this build has 1 */
+} pgx_tuning_t;
+
+typedef struct pgx_plan pgx_plan_t;
+
+/* One persistent launch over bucket b (the counter is reset on `stream` first).
+ * tuning NULL = the runner's autotuned shape for this kernel, or the default. */
+int pgx_test_persistent(pgx_runner_t* h, const pgx_bucket_t* b, const pgx_tuning_t* tuning,
+                        uint32_t flags, void* stream);
+
+/* Persistent launch over a device-resident work list: work item i tests column
+ * d_index[i] of the bucket, for i < *d_count (d_count read on the device, at
+ * most b->n).  Used for the extra Miller-Rabin bases on compacted survivors. */
+int pgx_test_indexed(pgx_runner_t* h, const pgx_bucket_t* b, const uint32_t* d_index,
+                     const uint32_t* d_count, const pgx_tuning_t* tuning, uint32_t flags,
+                     void* stream);
+
+/* Stream compaction on the device: appends every column i < n with
+ * d_verdict[i] != PGX_VERDICT_INVALID and (d_verdict[i] & mask) != 0 to
+ * d_index and its count to *d_count (reset first).  Order within the list is
+ * unspecified. */
+int pgx_compact(pgx_runner_t* h, const uint32_t* d_verdict, uint32_t n, uint32_t mask,
+                uint32_t* d_index, uint32_t* d_count, void* stream);
+
+/* Autotuner: times persistent launch shapes (blocks_per_cu x waves_per_block x
+ * chunk) on bucket b with `flags`, keeps the fastest for b's kernel in the
+ * runner (used when tuning is NULL) and returns it in *best with the measured
+ * candidates per second in *per_sec (either may be NULL). */
+int pgx_autotune(pgx_runner_t* h, const pgx_bucket_t* b, uint32_t flags, pgx_tuning_t* best,
+                 double* per_sec);
+
+/* Capture every bucket into one hipGraph (tuning[i] for bucket i, or NULL for
+ * the runner's tuned shapes).  The buffers the buckets point to must outlive
+ * the plan; each pgx_plan_launch re-tests them with their current contents. */
+int pgx_plan_create(pgx_plan_t** out, pgx_runner_t* h, const pgx_bucket_t* buckets,
+                    uint32_t nbuckets, const pgx_tuning_t* tuning, uint32_t flags);
+int pgx_plan_launch(pgx_plan_t* p, void* stream);
+int pgx_plan_destroy(pgx_plan_t* p);
+
 /* ---- candidate construction: N = m * K + X with K = P# / d ----------------
  * Host-side builder for synthetic sieved candidates, the role of fill_batch
  * (gpu_testing.cu:136-195) and init_K (gap_common.cpp:60-65).

@@ -11,19 +11,32 @@
 namespace pgx {
 
 // Arguments of one plain (one launch per batch) PRP launch.
+// Work item i tests column p = index ? index[i] : i of cand and writes column p
+// of verdict / residue.
 struct PrpArgs {
-  const uint32_t* cand;  // limb-interleaved: limb j of candidate i at cand[j * ld + i]
+  const uint32_t* cand;  // limb-interleaved: limb j of column p at cand[j * ld + p]
   uint32_t ld;
-  uint32_t n;
-  uint32_t buf_limbs;  // rows stored in cand (limbs beyond the kernel's S must be zero)
+  uint32_t n;          // work items (upper bound when count is set)
+  uint32_t buf_limbs;  // rows stored in cand (rows beyond the kernel's width must be zero)
   uint32_t mode;       // PGX_MODE_*
   Bases bases;
-  uint32_t* verdict;  // n words
-  uint32_t* residue;  // buf_limbs rows of res_ld words, or null
+  uint32_t* verdict;         // indexed by column
+  uint32_t* residue;         // buf_limbs rows of res_ld words, or null
   uint32_t res_ld;
+  const uint32_t* index;     // work item -> column, or null (identity)
+  const uint32_t* count;     // device-resident work-item count, or null (use n)
+};
+
+// Grid of a persistent launch.
+struct PersistShape {
+  uint32_t blocks;   // resident workgroups
+  uint32_t threads;  // threads per workgroup (multiple of 64, <= kBlock)
+  uint32_t chunk;    // work items a wavefront takes at a time (multiple of 64)
 };
 
 typedef hipError_t (*PrpLaunchFn)(const PrpArgs&, hipStream_t);
+typedef hipError_t (*PersistLaunchFn)(const PrpArgs&, uint32_t* counter, const PersistShape&,
+                                      hipStream_t);
 
 // Kernel limb counts: S radix-2^RB limbs hold candidates of up to
 // RB*S - 4 bits (RB = 29 up to 31 limbs, 28 above).
@@ -33,6 +46,7 @@ inline int kernel_capacity_bits(int S) { return (S <= 31 ? 29 : 28) * S - 4; }
 // Launchers indexed by kernel limb count S (1..kMaxKernelLimbs); an entry is
 // null when no translation unit instantiated that S.
 PrpLaunchFn* prp_table();
+PersistLaunchFn* persist_table();
 
 // Smallest instantiated S whose capacity holds max_bits, or 0.
 inline int kernel_for_bits(uint32_t max_bits) {

@@ -1,0 +1,287 @@
+// C ABI of the batched path: persistent launches, indexed (compacted) launches,
+// compaction, the launch-shape autotuner and hipGraph plans over buckets of
+// mixed candidate sizes.  See include/pgx.h for each entry point's contract.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "launch.hpp"
+#include "runtime.hpp"
+
+using pgx::fail;
+
+namespace {
+
+uint32_t tune_key(int S, uint32_t flags) { return ((uint32_t)S << 8) | (flags & PGX_MODE_MASK); }
+
+pgx_tuning_t default_tuning() {
+  pgx_tuning_t t{};
+  t.blocks_per_cu = 8;  // 32 wavefronts per CU: more than any kernel keeps resident
+  t.waves_per_block = 4;
+  t.chunk = 64;
+  t.lanes_per_number = 1;
+  return t;
+}
+
+int check_tuning(pgx_runner* h, const pgx_tuning_t& t) {
+  if (t.waves_per_block < 1 || t.waves_per_block > 4 || t.blocks_per_cu < 1 ||
+      t.blocks_per_cu > 64 || t.chunk < 64 || t.chunk % 64 || t.chunk > 4096)
+    return fail(h, PGX_E_ARG, "bad tuning: blocks_per_cu=%u waves_per_block=%u chunk=%u",
+                t.blocks_per_cu, t.waves_per_block, t.chunk);
+  if (t.lanes_per_number != 1)
+    return fail(h, PGX_E_ARG, "lanes_per_number=%u: only one lane per candidate is built",
+                t.lanes_per_number);
+  return PGX_OK;
+}
+
+// Validates a bucket; returns the kernel limb count (> 0) or a negative code.
+int check_bucket(pgx_runner* h, const pgx_bucket_t* b, uint32_t flags) {
+  if (!b || !b->d_cand || !b->d_verdict) return fail(h, PGX_E_ARG, "null bucket or pointer");
+  if (b->ld < b->n || b->limbs == 0)
+    return fail(h, PGX_E_ARG, "bad bucket shape: n=%u ld=%u limbs=%u", b->n, b->ld, b->limbs);
+  if ((flags & PGX_WANT_RESIDUE) && (!b->d_residues || b->res_ld < b->n))
+    return fail(h, PGX_E_ARG, "residues requested without a buffer of res_ld >= n");
+  if ((flags & PGX_MODE_MASK) > PGX_MODE_FERMAT) return fail(h, PGX_E_ARG, "bad mode");
+  if (b->max_bits < 2) return fail(h, PGX_E_ARG, "max_bits must be >= 2");
+  const int S = pgx::kernel_for_bits(b->max_bits);
+  if (S == 0 || !pgx::persist_table()[S])
+    return fail(h, PGX_E_INPUT, "candidates of %u bits exceed the widest kernel (%d bits)",
+                b->max_bits, PGX_MAX_BITS);
+  return S;
+}
+
+pgx::PrpArgs args_of(pgx_runner* h, const pgx_bucket_t* b, uint32_t flags) {
+  pgx::PrpArgs a{};
+  a.cand = b->d_cand;
+  a.ld = b->ld;
+  a.n = b->n;
+  a.buf_limbs = b->limbs;
+  a.mode = flags & PGX_MODE_MASK;
+  a.bases = h->bases;
+  a.verdict = b->d_verdict;
+  a.residue = (flags & PGX_WANT_RESIDUE) ? b->d_residues : nullptr;
+  a.res_ld = b->res_ld;
+  return a;
+}
+
+pgx::PersistShape shape_of(pgx_runner* h, const pgx_tuning_t& t) {
+  pgx::PersistShape sh{};
+  sh.blocks = (uint32_t)h->n_cu * t.blocks_per_cu;
+  sh.threads = 64u * t.waves_per_block;
+  sh.chunk = t.chunk;
+  return sh;
+}
+
+pgx_tuning_t tuning_for(pgx_runner* h, int S, uint32_t flags, const pgx_tuning_t* given) {
+  if (given) return *given;
+  auto it = h->tuned.find(tune_key(S, flags));
+  return it == h->tuned.end() ? default_tuning() : it->second;
+}
+
+// Next counter slot of the runner: launches in flight on different streams
+// (up to kCounters - 1 of them) never share a counter.
+uint32_t* next_counter(pgx_runner* h) {
+  static thread_local uint32_t slot = 0;
+  slot = (slot + 1) % (pgx::kCounters - 1);
+  return h->d_counters + 1 + slot;
+}
+
+int launch_persistent(pgx_runner* h, const pgx_bucket_t* b, const pgx_tuning_t* tuning,
+                      uint32_t flags, const uint32_t* d_index, const uint32_t* d_count,
+                      hipStream_t s) {
+  const int S = check_bucket(h, b, flags);
+  if (S < 0) return S;
+  const pgx_tuning_t t = tuning_for(h, S, flags, tuning);
+  int rc = check_tuning(h, t);
+  if (rc != PGX_OK) return rc;
+  if (b->n == 0) return PGX_OK;
+  pgx::PrpArgs a = args_of(h, b, flags);
+  a.index = d_index;
+  a.count = d_count;
+  uint32_t* counter = next_counter(h);
+  PGX_HIP(h, hipSetDevice(h->device));
+  PGX_HIP(h, pgx::launch_fill(counter, 1, 0u, s));
+  PGX_HIP(h, pgx::persist_table()[S](a, counter, shape_of(h, t), s));
+  return PGX_OK;
+}
+
+}  // namespace
+
+struct pgx_plan {
+  pgx_runner* h = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t* d_counters = nullptr;
+  std::vector<hipStream_t> streams;  // [0] captures, [1..] one per bucket
+  std::vector<hipEvent_t> events;
+};
+
+static void plan_release(pgx_plan* p) {
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
+  for (hipStream_t s : p->streams) (void)hipStreamDestroy(s);
+  if (p->d_counters) (void)hipFree(p->d_counters);
+}
+
+extern "C" {
+
+int pgx_test_persistent(pgx_runner_t* h, const pgx_bucket_t* b, const pgx_tuning_t* tuning,
+                        uint32_t flags, void* stream) {
+  if (!h) return PGX_E_ARG;
+  return launch_persistent(h, b, tuning, flags, nullptr, nullptr,
+                           stream ? (hipStream_t)stream : h->stream);
+}
+
+int pgx_test_indexed(pgx_runner_t* h, const pgx_bucket_t* b, const uint32_t* d_index,
+                     const uint32_t* d_count, const pgx_tuning_t* tuning, uint32_t flags,
+                     void* stream) {
+  if (!h) return PGX_E_ARG;
+  if (!d_index || !d_count) return fail(h, PGX_E_ARG, "null index list or count");
+  return launch_persistent(h, b, tuning, flags, d_index, d_count,
+                           stream ? (hipStream_t)stream : h->stream);
+}
+
+int pgx_compact(pgx_runner_t* h, const uint32_t* d_verdict, uint32_t n, uint32_t mask,
+                uint32_t* d_index, uint32_t* d_count, void* stream) {
+  if (!h) return PGX_E_ARG;
+  if (!d_verdict || !d_index || !d_count) return fail(h, PGX_E_ARG, "null device pointer");
+  PGX_HIP(h, hipSetDevice(h->device));
+  PGX_HIP(h, pgx::launch_compact(d_verdict, n, mask, d_index, d_count,
+                                 stream ? (hipStream_t)stream : h->stream));
+  return PGX_OK;
+}
+
+int pgx_autotune(pgx_runner_t* h, const pgx_bucket_t* b, uint32_t flags, pgx_tuning_t* best,
+                 double* per_sec) {
+  if (!h) return PGX_E_ARG;
+  const int S = check_bucket(h, b, flags);
+  if (S < 0) return S;
+  if (b->n == 0) return fail(h, PGX_E_ARG, "empty bucket");
+  // time on at most 2^18 candidates of the bucket
+  pgx_bucket_t sample = *b;
+  sample.n = std::min<uint32_t>(b->n, 1u << 18);
+  hipEvent_t e0, e1;
+  PGX_HIP(h, hipSetDevice(h->device));
+  PGX_HIP(h, hipEventCreate(&e0));
+  PGX_HIP(h, hipEventCreate(&e1));
+  pgx_tuning_t top = default_tuning();
+  float top_ms = 0;
+  int rc = PGX_OK;
+  const uint32_t waves[] = {1, 2, 4};
+  const uint32_t blocks[] = {1, 2, 3, 4, 6, 8, 12, 16};
+  const uint32_t chunks[] = {64, 256};
+  // one untimed launch first (code object load, clocks)
+  rc = launch_persistent(h, &sample, &top, flags, nullptr, nullptr, h->stream);
+  for (uint32_t w : waves)
+    for (uint32_t bl : blocks)
+      for (uint32_t ch : chunks) {
+        if (rc != PGX_OK) break;
+        if (w * bl > 32) continue;  // beyond the 32 wavefronts a CU can hold
+        pgx_tuning_t t{bl, w, ch, 1};
+        float best_ms = 0;
+        for (int rep = 0; rep < 2 && rc == PGX_OK; ++rep) {
+          hipError_t e = hipEventRecord(e0, h->stream);
+          if (e == hipSuccess) rc = launch_persistent(h, &sample, &t, flags, nullptr, nullptr, h->stream);
+          if (e == hipSuccess && rc == PGX_OK) e = hipEventRecord(e1, h->stream);
+          if (e == hipSuccess && rc == PGX_OK) e = hipEventSynchronize(e1);
+          float ms = 0;
+          if (e == hipSuccess && rc == PGX_OK) e = hipEventElapsedTime(&ms, e0, e1);
+          if (e != hipSuccess) rc = fail(h, PGX_E_HIP, "autotune timing: %s", hipGetErrorString(e));
+          if (rc == PGX_OK && (best_ms == 0 || ms < best_ms)) best_ms = ms;
+        }
+        if (rc == PGX_OK && (top_ms == 0 || best_ms < top_ms)) {
+          top_ms = best_ms;
+          top = t;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != PGX_OK) return rc;
+  h->tuned[tune_key(S, flags)] = top;
+  if (best) *best = top;
+  if (per_sec) *per_sec = top_ms > 0 ? sample.n / (top_ms * 1e-3) : 0.0;
+  return PGX_OK;
+}
+
+int pgx_plan_create(pgx_plan_t** out, pgx_runner_t* h, const pgx_bucket_t* buckets,
+                    uint32_t nbuckets, const pgx_tuning_t* tuning, uint32_t flags) {
+  if (!out || !h) return PGX_E_ARG;
+  *out = nullptr;
+  if (!buckets || nbuckets == 0 || nbuckets > 64)
+    return fail(h, PGX_E_ARG, "nbuckets must be in [1, 64]");
+  std::vector<int> S(nbuckets);
+  std::vector<pgx_tuning_t> T(nbuckets);
+  for (uint32_t k = 0; k < nbuckets; ++k) {
+    S[k] = check_bucket(h, &buckets[k], flags);
+    if (S[k] < 0) return S[k];
+    T[k] = tuning_for(h, S[k], flags, tuning ? &tuning[k] : nullptr);
+    int rc = check_tuning(h, T[k]);
+    if (rc != PGX_OK) return rc;
+  }
+  PGX_HIP(h, hipSetDevice(h->device));
+  pgx_plan* p = new (std::nothrow) pgx_plan();
+  if (!p) return fail(h, PGX_E_ARG, "out of host memory");
+  p->h = h;
+  hipError_t e = hipMalloc((void**)&p->d_counters, nbuckets * 4);
+  for (uint32_t k = 0; k <= nbuckets && e == hipSuccess; ++k) {
+    hipStream_t s = nullptr;
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) p->streams.push_back(s);
+  }
+  for (uint32_t k = 0; k <= nbuckets && e == hipSuccess; ++k) {
+    hipEvent_t ev = nullptr;
+    e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) p->events.push_back(ev);
+  }
+  if (e != hipSuccess) {
+    plan_release(p);
+    delete p;
+    return fail(h, PGX_E_HIP, "plan resources: %s", hipGetErrorString(e));
+  }
+  // capture: fork from streams[0], one branch per bucket, join back
+  hipStream_t cap = p->streams[0];
+  e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) e = hipEventRecord(p->events[0], cap);
+  for (uint32_t k = 0; k < nbuckets && e == hipSuccess; ++k) {
+    hipStream_t s = p->streams[k + 1];
+    e = hipStreamWaitEvent(s, p->events[0], 0);
+    if (e == hipSuccess) e = pgx::launch_fill(p->d_counters + k, 1, 0u, s);
+    if (e == hipSuccess && buckets[k].n > 0) {
+      pgx::PrpArgs a = args_of(h, &buckets[k], flags);
+      e = pgx::persist_table()[S[k]](a, p->d_counters + k, shape_of(h, T[k]), s);
+    }
+    if (e == hipSuccess) e = hipEventRecord(p->events[k + 1], s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(cap, p->events[k + 1], 0);
+  }
+  hipError_t e_end = hipStreamEndCapture(cap, &p->graph);
+  if (e == hipSuccess) e = e_end;
+  if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    plan_release(p);
+    delete p;
+    return fail(h, PGX_E_HIP, "plan capture: %s", hipGetErrorString(e));
+  }
+  *out = p;
+  return PGX_OK;
+}
+
+int pgx_plan_launch(pgx_plan_t* p, void* stream) {
+  if (!p) return PGX_E_ARG;
+  PGX_HIP(p->h, hipSetDevice(p->h->device));
+  PGX_HIP(p->h, hipGraphLaunch(p->exec, stream ? (hipStream_t)stream : p->h->stream));
+  return PGX_OK;
+}
+
+int pgx_plan_destroy(pgx_plan_t* p) {
+  if (!p) return PGX_E_ARG;
+  (void)hipSetDevice(p->h->device);
+  (void)hipDeviceSynchronize();
+  plan_release(p);
+  delete p;
+  return PGX_OK;
+}
+
+}  // extern "C"

@@ -11,6 +11,7 @@
 
 #include "../../include/pgx.h"
 #include "launch.hpp"
+#include "runtime.hpp"
 
 namespace pgx {
 
@@ -19,53 +20,21 @@ PrpLaunchFn* prp_table() {
   return table;
 }
 
-}  // namespace pgx
-
-struct pgx_runner {
-  int device = 0;
-  size_t max_n = 0;
-  uint32_t limbs = 0;
-  pgx::Bases bases{};
-  hipStream_t stream = nullptr;
-  // pinned staging (limb-interleaved, ld = max_n)
-  uint32_t* h_cand = nullptr;
-  uint32_t* h_verdict = nullptr;
-  uint32_t* h_res = nullptr;
-  // device buffers
-  uint32_t* d_cand = nullptr;
-  uint32_t* d_verdict = nullptr;
-  uint32_t* d_res = nullptr;
-  // pending batch
-  bool pending = false;
-  size_t pending_n = 0;
-  uint32_t pending_flags = 0;
-  std::string err;
-};
-
-namespace {
-
-std::string g_create_err = "no error";
-
-int fail(pgx_runner* h, int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  if (h)
-    h->err = buf;
-  else
-    g_create_err = buf;
-  return code;
+PersistLaunchFn* persist_table() {
+  static PersistLaunchFn table[kMaxKernelLimbs + 1] = {};
+  return table;
 }
 
-#define PGX_HIP(h, call)                                                                   \
-  do {                                                                                     \
-    hipError_t e_ = (call);                                                                \
-    if (e_ != hipSuccess)                                                                  \
-      return fail((h), PGX_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                  __FILE__, __LINE__);                                                     \
-  } while (0)
+std::string& create_error() {
+  static std::string msg = "no error";
+  return msg;
+}
+
+}  // namespace pgx
+
+using pgx::fail;
+
+namespace {
 
 // First `count` primes, the reference's generate_primes (miller_rabin.h:272-289).
 void first_primes(uint32_t count, uint32_t* out) {
@@ -95,6 +64,7 @@ void release(pgx_runner* h) {
   if (h->d_cand) (void)hipFree(h->d_cand);
   if (h->d_verdict) (void)hipFree(h->d_verdict);
   if (h->d_res) (void)hipFree(h->d_res);
+  if (h->d_counters) (void)hipFree(h->d_counters);
 }
 
 int launch(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs, size_t n,
@@ -163,6 +133,9 @@ int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t lim
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_cand, words * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_verdict, max_n * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_res, words * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&h->d_counters, pgx::kCounters * 4);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     release(h);
@@ -265,7 +238,7 @@ int pgx_test_device(pgx_runner_t* h, const uint32_t* d_cand, uint32_t ld, uint32
 }
 
 const char* pgx_last_error(const pgx_runner_t* h) {
-  return h ? h->err.c_str() : g_create_err.c_str();
+  return h ? h->err.c_str() : pgx::create_error().c_str();
 }
 
 }  // extern "C"

@@ -18,12 +18,11 @@
 
 namespace pgx {
 
+// Tests the candidate in column p of a.cand and writes column p of the outputs.
 template <int S, bool kBase2Only>
-__global__ __launch_bounds__(kBlock) void k_prp_plain(PrpArgs a) {
+__device__ __forceinline__ void test_column(const PrpArgs& a, uint32_t p) {
   constexpr int W = Radix<S>::W32;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  const uint32_t* c = a.cand + i;
+  const uint32_t* c = a.cand + p;
   const size_t ld = a.ld;
   uint32_t w[W];
 #pragma unroll
@@ -38,14 +37,48 @@ __global__ __launch_bounds__(kBlock) void k_prp_plain(PrpArgs a) {
 #pragma unroll
     for (int j = 0; j < W; ++j) rw[j] = 0;
   }
-  a.verdict[i] = v;
+  a.verdict[p] = v;
   if (a.residue) {
-    uint32_t* o = a.residue + i;
+    uint32_t* o = a.residue + p;
     const size_t rld = a.res_ld;
 #pragma unroll
     for (int j = 0; j < W; ++j)
       if (j < (int)a.buf_limbs) o[j * rld] = rw[j];
     for (uint32_t j = W; j < a.buf_limbs; ++j) o[j * rld] = 0u;
+  }
+}
+
+// Number of work items: a.n, or *a.count when the count lives on the device
+// (then a.n is only the bound the grid was sized for).
+__device__ __forceinline__ uint32_t work_items(const PrpArgs& a) {
+  return a.count ? min(*a.count, a.n) : a.n;
+}
+
+// One launch per batch: thread i tests work item i.
+template <int S, bool kBase2Only>
+__global__ __launch_bounds__(kBlock) void k_prp_plain(PrpArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= work_items(a)) return;
+  test_column<S, kBase2Only>(a, a.index ? a.index[i] : i);
+}
+
+// Persistent: a fixed grid of resident wavefronts takes `chunk` work items at a
+// time (chunk a multiple of 64) from *counter until the batch is exhausted.
+// The counter only grows, so every wavefront reaches the exit.
+template <int S, bool kBase2Only>
+__global__ __launch_bounds__(kBlock) void k_prp_persistent(PrpArgs a, uint32_t* counter,
+                                                           uint32_t chunk) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n = work_items(a);
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(counter, chunk);
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (base >= n) break;
+    for (uint32_t off = lane; off < chunk; off += 64) {
+      const uint32_t i = base + off;
+      if (i < n) test_column<S, kBase2Only>(a, a.index ? a.index[i] : i);
+    }
   }
 }
 
@@ -59,9 +92,24 @@ hipError_t launch_plain(const PrpArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+template <int S>
+hipError_t launch_persistent(const PrpArgs& a, uint32_t* counter, const PersistShape& sh,
+                             hipStream_t stream) {
+  if (a.mode == PGX_MODE_FERMAT)
+    hipLaunchKernelGGL((k_prp_persistent<S, true>), dim3(sh.blocks), dim3(sh.threads), 0, stream,
+                       a, counter, sh.chunk);
+  else
+    hipLaunchKernelGGL((k_prp_persistent<S, false>), dim3(sh.blocks), dim3(sh.threads), 0,
+                       stream, a, counter, sh.chunk);
+  return hipGetLastError();
+}
+
 template <int... Ss>
 struct Register {
-  Register() { ((prp_table()[Ss] = &launch_plain<Ss>), ...); }
+  Register() {
+    ((prp_table()[Ss] = &launch_plain<Ss>), ...);
+    ((persist_table()[Ss] = &launch_persistent<Ss>), ...);
+  }
 };
 
 static const Register<PGX_S_VALUES> registered;

@@ -67,7 +67,31 @@ EXPORTED = (
     "pgx_gen_multipliers",
     "pgx_k_bits",
     "pgx_valu_rate",
+    "pgx_test_persistent",
+    "pgx_test_indexed",
+    "pgx_compact",
+    "pgx_autotune",
+    "pgx_plan_create",
+    "pgx_plan_launch",
+    "pgx_plan_destroy",
 )
+
+
+class Bucket(ctypes.Structure):
+    """pgx_bucket_t: one group of candidates tested by one kernel width."""
+    _fields_ = [("d_cand", ctypes.c_void_p), ("ld", ctypes.c_uint32), ("limbs", ctypes.c_uint32),
+                ("n", ctypes.c_uint32), ("max_bits", ctypes.c_uint32),
+                ("d_verdict", ctypes.c_void_p), ("d_residues", ctypes.c_void_p),
+                ("res_ld", ctypes.c_uint32)]
+
+
+class Tuning(ctypes.Structure):
+    """pgx_tuning_t: persistent launch shape."""
+    _fields_ = [("blocks_per_cu", ctypes.c_uint32), ("waves_per_block", ctypes.c_uint32),
+                ("chunk", ctypes.c_uint32), ("lanes_per_number", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 class PgxError(RuntimeError):
@@ -103,6 +127,15 @@ def _load() -> ctypes.CDLL:
     lib.pgx_k_bits.restype = ctypes.c_uint32
     lib.pgx_valu_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.POINTER(ctypes.c_double)]
+    bp, tp = ctypes.POINTER(Bucket), ctypes.POINTER(Tuning)
+    lib.pgx_test_persistent.argtypes = [vp, bp, tp, ctypes.c_uint32, vp]
+    lib.pgx_test_indexed.argtypes = [vp, bp, vp, vp, tp, ctypes.c_uint32, vp]
+    lib.pgx_compact.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
+    lib.pgx_autotune.argtypes = [vp, bp, ctypes.c_uint32, tp, ctypes.POINTER(ctypes.c_double)]
+    lib.pgx_plan_create.argtypes = [ctypes.POINTER(vp), vp, bp, ctypes.c_uint32, tp,
+                                    ctypes.c_uint32]
+    lib.pgx_plan_launch.argtypes = [vp, vp]
+    lib.pgx_plan_destroy.argtypes = [vp]
     return lib
 
 
@@ -270,6 +303,33 @@ class Runner:
         for i in range(to_run):
             results[i] = int(v[i] == self.rounds)
 
+    def test_persistent(self, bucket: Bucket, mode: int, residues: bool = False,
+                        tuning: Tuning | None = None, stream: int = 0) -> None:
+        flags = mode | (WANT_RESIDUE if residues else 0)
+        self._check(lib.pgx_test_persistent(self._h, ctypes.byref(bucket),
+                                            ctypes.byref(tuning) if tuning else None, flags,
+                                            ctypes.c_void_p(stream or None)))
+
+    def test_indexed(self, bucket: Bucket, index_ptr: int, count_ptr: int, mode: int,
+                     tuning: Tuning | None = None, stream: int = 0) -> None:
+        self._check(lib.pgx_test_indexed(self._h, ctypes.byref(bucket), ctypes.c_void_p(index_ptr),
+                                         ctypes.c_void_p(count_ptr),
+                                         ctypes.byref(tuning) if tuning else None, mode,
+                                         ctypes.c_void_p(stream or None)))
+
+    def compact(self, verdict_ptr: int, n: int, mask: int, index_ptr: int, count_ptr: int,
+                stream: int = 0) -> None:
+        self._check(lib.pgx_compact(self._h, ctypes.c_void_p(verdict_ptr), n, mask,
+                                    ctypes.c_void_p(index_ptr), ctypes.c_void_p(count_ptr),
+                                    ctypes.c_void_p(stream or None)))
+
+    def autotune(self, bucket: Bucket, mode: int) -> tuple[Tuning, float]:
+        t = Tuning()
+        rate = ctypes.c_double(0)
+        self._check(lib.pgx_autotune(self._h, ctypes.byref(bucket), mode, ctypes.byref(t),
+                                     ctypes.byref(rate)))
+        return t, rate.value
+
     def test_device(self, cand_ptr: int, ld: int, limbs: int, n: int, max_bits: int,
                     mode: int, verdict_ptr: int, residue_ptr: int = 0, res_ld: int = 0,
                     stream: int = 0) -> None:
@@ -279,3 +339,63 @@ class Runner:
                                         max_bits, flags, ctypes.c_void_p(verdict_ptr),
                                         ctypes.c_void_p(residue_ptr or None), res_ld,
                                         ctypes.c_void_p(stream or None)))
+
+
+class Plan:
+    """pgx_plan_t: every bucket captured into one hipGraph launch."""
+
+    def __init__(self, runner: Runner, buckets, mode: int, residues: bool = False,
+                 tunings=None):
+        self.runner = runner
+        arr = (Bucket * len(buckets))(*buckets)
+        tarr = (Tuning * len(buckets))(*tunings) if tunings else None
+        h = ctypes.c_void_p()
+        flags = mode | (WANT_RESIDUE if residues else 0)
+        runner._check(lib.pgx_plan_create(ctypes.byref(h), runner._h, arr, len(buckets), tarr,
+                                          flags))
+        self._p = h
+        self._keep = (arr, tarr)
+
+    def launch(self, stream: int = 0) -> None:
+        self.runner._check(lib.pgx_plan_launch(self._p, ctypes.c_void_p(stream or None)))
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib.pgx_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+
+class DeviceBatch:
+    """Candidates resident in HBM (torch-owned), limb-interleaved, with verdict
+    and residue buffers: the device side of a pgx_bucket_t."""
+
+    def __init__(self, cands: np.ndarray, max_bits: int | None = None, ld: int | None = None,
+                 residues: bool = True):
+        import torch as _torch
+        cands = np.ascontiguousarray(cands, dtype=np.uint32)
+        self.n, self.limbs = cands.shape
+        self.ld = ld or self.n
+        if max_bits is None:
+            nz = [int.bit_length(int(x)) for x in from_limbs(cands)] if self.n else [2]
+            max_bits = max(max(nz), 2)
+        self.max_bits = int(max_bits)
+        host = np.zeros((self.limbs, self.ld), dtype=np.uint32)
+        host[:, :self.n] = cands.T
+        self.cand = _torch.from_numpy(host.view(np.int32)).cuda()
+        self.verdict = _torch.zeros(max(self.n, 1), dtype=_torch.int32, device="cuda")
+        self.res = (_torch.zeros((self.limbs, self.ld), dtype=_torch.int32, device="cuda")
+                    if residues else None)
+
+    def bucket(self) -> Bucket:
+        return Bucket(self.cand.data_ptr(), self.ld, self.limbs, self.n, self.max_bits,
+                      self.verdict.data_ptr(), self.res.data_ptr() if self.res is not None else 0,
+                      self.ld)
+
+    def verdicts(self) -> np.ndarray:
+        return self.verdict.cpu().numpy().view(np.uint32)[:self.n].copy()
+
+    def residues(self) -> np.ndarray:
+        return self.res.cpu().numpy().view(np.uint32)[:, :self.n].T.copy()

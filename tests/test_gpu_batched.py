@@ -1,0 +1,159 @@
+"""GPU parity of the batched path: persistent kernels, hipGraph plans over
+mixed-size buckets, device compaction and the extra Miller-Rabin bases on
+Fermat survivors.  Checked against the plain kernel and the GMP oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pgx
+
+pytestmark = pytest.mark.gpu
+
+# (P, d, buffer words) of the mixed-size configuration (d = 210)
+MIXED = [(211, 210, 16), (337, 210, 16), (503, 210, 24), (701, 210, 32)]
+
+
+def say(msg):
+    print(f"[gpu-batched] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def gen(P, d, words, n_m, per_m, seed_off=0):
+    m = pgx.gen_multipliers(0x5EED0100 + P + seed_off, 10**7, 10**7 + 10**4, d, n_m)
+    L, _, _ = pgx.gen_sieved(P, d, m, 450000, per_m, words)
+    return L
+
+
+def test_persistent_equals_plain_and_oracle():
+    import torch
+    with pgx.Runner(1, limbs=16) as run:
+        for P, d, words in MIXED:
+            L = gen(P, d, words, 8, 256)
+            say(f"persistent P={P}: {len(L)} candidates, {words} words")
+            a = pgx.DeviceBatch(L)
+            run.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True)
+            torch.cuda.synchronize()
+            ov, ores = O.verdicts(L, O.MODE_FERMAT, residues=True)
+            assert np.array_equal(a.verdicts(), ov)
+            assert np.array_equal(a.residues(), ores)
+            # a non-default shape gives the same answer
+            t = pgx.Tuning(2, 1, 256, 1)
+            b = pgx.DeviceBatch(L)
+            run.test_persistent(b.bucket(), pgx.MODE_FERMAT, residues=True, tuning=t)
+            torch.cuda.synchronize()
+            assert np.array_equal(b.verdicts(), ov)
+
+
+def test_plan_graph_replays_match_eager():
+    """One hipGraph over four buckets; replayed three times with eager work and
+    synchronizations in between, every replay equal to the eager result."""
+    import torch
+    batches, eager = [], []
+    with pgx.Runner(1, limbs=16) as run:
+        for P, d, words in MIXED:
+            L = gen(P, d, words, 4, 256, seed_off=7)
+            batches.append(pgx.DeviceBatch(L))
+            e = pgx.DeviceBatch(L)
+            run.test_persistent(e.bucket(), pgx.MODE_FERMAT, residues=True)
+            eager.append(e)
+        torch.cuda.synchronize()
+        say(f"plan over {len(batches)} buckets")
+        plan = pgx.Plan(run, [b.bucket() for b in batches], pgx.MODE_FERMAT, residues=True)
+        for rep in range(3):
+            for b in batches:  # clear outputs between replays
+                b.verdict.zero_()
+                b.res.zero_()
+            torch.cuda.synchronize()
+            plan.launch()
+            torch.cuda.synchronize()
+            for b, e in zip(batches, eager):
+                assert np.array_equal(b.verdicts(), e.verdicts()), rep
+                assert np.array_equal(b.residues(), e.residues()), rep
+            # eager work between replays
+            run.test_persistent(eager[0].bucket(), pgx.MODE_FERMAT, residues=True)
+            torch.cuda.synchronize()
+        plan.close()
+    # and the eager result itself is the oracle's
+    L0 = gen(*MIXED[0], 4, 256, seed_off=7)
+    assert np.array_equal(eager[0].verdicts(), O.verdicts(L0, O.MODE_FERMAT))
+
+
+def test_fermat_then_mr_on_compacted_survivors():
+    """Base-2 Fermat on every candidate, on-device compaction of the PRPs, then
+    strong tests to bases 3 and 5 on the survivors only (P=907, d=30030)."""
+    import torch
+    L = gen(907, 30030, 64, 16, 128)
+    n = len(L)
+    say(f"fermat->compact->MR: {n} candidates of ~1260 bits")
+    a = pgx.DeviceBatch(L)
+    index = torch.zeros(n, dtype=torch.int32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int32, device="cuda")
+    mr = pgx.DeviceBatch(L, residues=False)  # same candidates; MR verdicts land here
+    mr.verdict.fill_(-7)                       # marker: untouched columns keep it
+    # one stream orders the three steps (each runner's default stream is its own)
+    s = torch.cuda.Stream().cuda_stream
+    with pgx.Runner(1, limbs=64) as fermat, pgx.Runner(1, limbs=64, bases=[3, 5]) as extra:
+        fermat.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True, stream=s)
+        fermat.compact(a.verdict.data_ptr(), n, pgx.FLAG_FERMAT, index.data_ptr(),
+                       count.data_ptr(), stream=s)
+        extra.test_indexed(mr.bucket(), index.data_ptr(), count.data_ptr(), pgx.MODE_STRONG,
+                           stream=s)
+        torch.cuda.synchronize()
+    v = a.verdicts()
+    ov = O.verdicts(L, O.MODE_FERMAT)
+    assert np.array_equal(v, ov)
+    surv = np.nonzero(ov & 1)[0]
+    k = int(count.item())
+    assert k == len(surv) and k > 0
+    assert sorted(index.cpu().numpy()[:k].tolist()) == surv.tolist()
+    got = mr.verdicts()
+    want = O.verdicts(L[surv], O.MODE_STRONG, (3, 5))
+    assert np.array_equal(got[surv], want)
+    others = np.setdiff1d(np.arange(n), surv)
+    assert np.all(got[others] == np.uint32(-7 & 0xFFFFFFFF))
+
+
+def test_indexed_with_empty_list():
+    import torch
+    L = gen(337, 2310, 16, 1, 64)
+    b = pgx.DeviceBatch(L, residues=False)
+    b.verdict.fill_(5)
+    index = torch.zeros(1, dtype=torch.int32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int32, device="cuda")
+    with pgx.Runner(1, limbs=16) as run:
+        run.test_indexed(b.bucket(), index.data_ptr(), count.data_ptr(), pgx.MODE_FERMAT)
+        torch.cuda.synchronize()
+    assert np.all(b.verdicts() == 5)
+
+
+def test_autotune_picks_a_valid_shape():
+    import torch
+    L = gen(337, 2310, 16, 16, 1024)
+    b = pgx.DeviceBatch(L)
+    with pgx.Runner(1, limbs=16) as run:
+        t, rate = run.autotune(b.bucket(), pgx.MODE_FERMAT)
+        say(f"autotune: {t.as_dict()} {rate / 1e6:.1f} M/s")
+        assert rate > 0 and t.lanes_per_number == 1 and t.chunk % 64 == 0
+        b.verdict.zero_()
+        run.test_persistent(b.bucket(), pgx.MODE_FERMAT, residues=True)  # uses the tuned shape
+        torch.cuda.synchronize()
+    ov, ores = O.verdicts(L[:4096], O.MODE_FERMAT, residues=True)
+    assert np.array_equal(b.verdicts()[:4096], ov)
+    assert np.array_equal(b.residues()[:4096], ores)
+
+
+def test_bad_tuning_and_bucket_are_rejected():
+    L = gen(337, 2310, 16, 1, 64)
+    b = pgx.DeviceBatch(L)
+    with pgx.Runner(1, limbs=16) as run:
+        for bad in (pgx.Tuning(1, 1, 100, 1), pgx.Tuning(1, 5, 64, 1), pgx.Tuning(1, 1, 64, 4)):
+            with pytest.raises(pgx.PgxError) as e:
+                run.test_persistent(b.bucket(), pgx.MODE_FERMAT, tuning=bad)
+            assert e.value.code == pgx.PGX_E_ARG
+        wide = b.bucket()
+        wide.max_bits = 5000
+        with pytest.raises(pgx.PgxError) as e:
+            run.test_persistent(wide, pgx.MODE_FERMAT)
+        assert e.value.code == pgx.PGX_E_INPUT
