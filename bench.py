@@ -199,6 +199,82 @@ def bench_large(stream, peak: float, reps: int) -> dict:
     }
 
 
+def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: int = 100) -> dict:
+    """Poisson stream of per-m next/prev-prime requests (P in {337, 503, 907},
+    d=2310, 20-500 sieved X per side), continuous batching under a 64 GB cap.
+    Each request is submitted at its scheduled arrival time; latency runs from
+    that time to completion, so driver lateness counts against the engine."""
+    import oracle
+    import pgx
+    sizes = [(337, 16), (503, 24), (907, 48)]
+    t0 = time.time()
+    tmpl = []
+    for P, words in sizes:
+        bits = pgx.k_bits(P, 2310) + 25  # m < 2^24, plus the X carry
+        for m, nxt, prv in pgx.gen_requests(P, 2310, templates_per_p, SEED + 5 * P, words):
+            tmpl.append((P, bits, nxt, prv))
+    log(f"serving: {len(tmpl)} request templates in {time.time() - t0:.1f}s")
+    # expected first PRP per side for a sample of templates (GMP oracle)
+    rng = np.random.default_rng(SEED)
+    check = set(rng.choice(len(tmpl), size=24, replace=False).tolist())
+
+    def first(limbs):
+        v = oracle.verdicts(limbs, oracle.MODE_REFERENCE, (2,))
+        hit = np.nonzero(v == 1)[0]
+        return int(hit[0]) if len(hit) else -1
+    want = {k: (first(tmpl[k][2]), first(tmpl[k][3])) for k in check}
+
+    n_req = int(rate * duration_s)
+    gaps = rng.exponential(1.0 / rate, size=n_req)
+    pick = rng.integers(0, len(tmpl), size=n_req)
+    results, lat = {}, []
+    with pgx.Server(hbm_budget=64 << 30, chunk=32) as srv:
+        for k in range(len(sizes)):  # warm each width class (allocation, code load)
+            P, bits, nxt, prv = tmpl[k * templates_per_p]
+            srv.submit(10**9 + k, nxt, prv, bits)
+        warm = 0
+        t_warm = time.time()
+        while warm < len(sizes):
+            warm += len(srv.poll())
+            if time.time() - t_warm > 60:
+                raise RuntimeError(f"serving warm-up did not finish: {srv.stats()}")
+        start = time.monotonic_ns()
+        arrivals = start + np.cumsum(gaps * 1e9).astype(np.int64)
+        i = 0
+        last = time.monotonic_ns()
+        while len(results) < n_req:
+            now = time.monotonic_ns()
+            if now - last > 30e9:
+                raise RuntimeError(f"serving made no progress for 30 s: {srv.stats()}")
+            while i < n_req and arrivals[i] <= now:
+                P, bits, nxt, prv = tmpl[pick[i]]
+                srv.submit(i, nxt, prv, bits, arrival_ns=int(arrivals[i]))
+                i += 1
+            for rid, nf, pf, tests, lns in srv.poll():
+                last = now
+                if rid < n_req:
+                    results[rid] = (nf, pf, tests)
+                    lat.append(lns)
+        wall = (time.monotonic_ns() - start) * 1e-9
+        tests = sum(r[2] for r in results.values())
+    lat = np.array(lat) * 1e-6
+    consistent = all(results[r][:2] == want[pick[r]] for r in results if pick[r] in want)
+    by_t = {}
+    for r, v in results.items():
+        by_t.setdefault(int(pick[r]), set()).add(v[:2])
+    consistent &= all(len(v) == 1 for v in by_t.values())
+    return {
+        "workload": "Poisson per-m requests at 2e4/s for 1 s, P in {337,503,907}, d=2310, "
+                    "20-500 sieved X per side, next+prev search with early exit, 64 GB cap, "
+                    f"{len(tmpl)} request templates",
+        "requests": n_req, "wall_s": wall,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "max_ms": float(lat.max()),
+        "tests": tests, "tests_per_s": tests / wall,
+        "results_match_oracle_and_repeat": bool(consistent),
+    }
+
+
 def cpu_baseline(cands: np.ndarray, target_cpu_s: float = 15.0) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the GMP path, timed as the reported CPU baseline
@@ -315,6 +391,11 @@ def main() -> int:
     if not args.no_extras:
         extra["mixed_graph"] = bench_mixed(stream, peak, max(2, args.steps // 4))
         extra["large_fermat_mr"] = bench_large(stream, peak, max(2, args.steps // 4))
+        srv = bench_serving()
+        srv["plain_512_tests_per_s"] = tests_per_s
+        extra["serving"] = srv
+        log(f"serving: p50 {srv['p50_ms']:.2f} ms p99 {srv['p99_ms']:.2f} ms, "
+            f"{srv['tests_per_s'] / 1e6:.2f} M tests/s")
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -39,32 +39,7 @@
 extern "C" {
 #endif
 
-/* status codes */
-#define PGX_OK 0
-#define PGX_E_ARG (-1)      /* bad argument (null pointer, zero size, ...) */
-#define PGX_E_HIP (-2)      /* HIP runtime failure (message names the call) */
-#define PGX_E_CAPACITY (-3) /* n larger than the runner's max_n (miller_rabin.h:368-371) */
-#define PGX_E_INPUT (-4)    /* candidate even, <= 1, or too wide (miller_rabin.h:46-49, 375-376) */
-#define PGX_E_STATE (-5)    /* wait without submit, submit while busy */
-#define PGX_E_NODEV (-6)    /* no usable gfx950 device */
-
-/* test modes (flags & PGX_MODE_MASK) */
-#define PGX_MODE_REFERENCE 0u /* reference verdict: passed = number of leading bases that pass
-                                 the strong test as written in miller_rabin.h:216-248 (which
-                                 also accepts a^(N-1) == -1 when ctz(N-1) >= 2) */
-#define PGX_MODE_STRONG 1u    /* textbook strong test per base: passed = leading bases passed */
-#define PGX_MODE_FERMAT 2u    /* base-2 Fermat only: verdict word = PGX_FLAG_* bits of base 2 */
-#define PGX_MODE_MASK 3u
-#define PGX_WANT_RESIDUE 4u   /* also return a^(N-1) mod N of the first base (usually 2) */
-
-/* bits of the verdict word in PGX_MODE_FERMAT */
-#define PGX_FLAG_FERMAT 1u /* 2^(N-1) == 1 (mod N) */
-#define PGX_FLAG_SPRP 2u   /* strong probable prime to base 2 */
-#define PGX_FLAG_QUIRK 4u  /* reference verdict for base 2 */
-
-#define PGX_VERDICT_INVALID 0xFFFFFFFFu /* candidate rejected by the kernel */
-#define PGX_MAX_BASES 16
-#define PGX_MAX_BITS 2068 /* widest candidate the kernels take (74 radix-2^28 limbs) */
+#include "pgx_defs.h"
 
 typedef struct pgx_runner pgx_runner_t;
 
@@ -187,6 +162,13 @@ int pgx_gen_sieved(uint32_t P, uint32_t d, const uint64_t* m, uint32_t n_m, uint
                    uint32_t per_m, uint32_t sieve_limit, uint32_t limbs, uint32_t* out_limbs,
                    uint32_t* out_m, uint32_t* out_x, size_t* written);
 
+/* Same for either side of m*K: N = m*K + sign*X, sign = +1 (the next-prime
+ * side) or -1 (the prev-prime side, the reference's mpz_prevprime direction,
+ * overflow.cpp:158). */
+int pgx_gen_sieved_side(uint32_t P, uint32_t d, int32_t sign, const uint64_t* m, uint32_t n_m,
+                        uint32_t x_max, uint32_t per_m, uint32_t sieve_limit, uint32_t limbs,
+                        uint32_t* out_limbs, uint32_t* out_m, uint32_t* out_x, size_t* written);
+
 /* Draw n_m distinct multipliers uniformly from [m_lo, m_hi) coprime to d with
  * the generator's seeded xoshiro256** stream (rejection sampling, ascending
  * order on return). */
@@ -205,6 +187,62 @@ uint32_t pgx_k_bits(uint32_t P, uint32_t d);
  * 3 v_mul_lo_u32, 4 v_fma_f64.  Gives the denominator of the INT32
  * multiply-add roofline. */
 int pgx_valu_rate(int device, int op, uint32_t blocks, uint32_t iters, double* per_sec);
+
+/* ---- serving: next-prime / prev-prime searches, continuous batching ------
+ * Per-m requests in the role of the reference's overflow path
+ * (mpz_nextprime / mpz_prevprime, overflow.cpp:148, 158) and its found-prime
+ * bookkeeping (gpu_testing.cu:104-133): every request carries the sieved
+ * candidates of both sides of m*K in ascending X; the answer per side is the
+ * first candidate passing the reference's verdict with base 2 (early exit).
+ * Candidates stay in HBM pools, one per kernel width; requests are admitted
+ * while the byte budget and pool room allow and join the next iteration of
+ * their width class; classes run on their own streams and overlap.  No
+ * counterpart exists in the reference (which has no serving path). */
+typedef struct {
+  uint64_t hbm_budget;   /* bytes of admitted candidate + verdict storage */
+  uint32_t chunk;        /* candidates per side per iteration */
+  uint32_t max_requests; /* admitted requests per width class */
+  uint32_t pool_columns; /* candidate columns per width class pool */
+  uint32_t max_side;     /* most candidates one side may carry */
+  uint32_t stage_columns;/* columns admitted per class per iteration (pinned staging) */
+} pgx_serve_config_t;
+
+typedef struct {
+  uint64_t id;
+  uint64_t arrival_ns;   /* CLOCK_MONOTONIC ns of arrival (0 = at submit) */
+  uint32_t limbs;        /* 32-bit words per candidate in next/prev */
+  uint32_t max_bits;     /* bound on every candidate's bit length */
+  uint32_t n_next, n_prev;
+  const uint32_t* next;  /* n_next candidates m*K + X, ascending X, candidate-major */
+  const uint32_t* prev;  /* n_prev candidates m*K - X, ascending X */
+} pgx_serve_request_t;
+
+typedef struct {
+  uint64_t id;
+  int32_t next_first;    /* index into next of the first PRP, -1 if none */
+  int32_t prev_first;
+  uint32_t tests;        /* candidates tested for this request */
+  uint64_t latency_ns;   /* arrival to completion */
+} pgx_serve_result_t;
+
+typedef struct {
+  uint64_t bytes_in_use;
+  uint64_t tests;
+  uint32_t active, waiting, in_flight, finished_unread;
+} pgx_serve_stats_t;
+
+typedef struct pgx_server pgx_server_t;
+
+int pgx_serve_create(pgx_server_t** out, int device, const pgx_serve_config_t* cfg);
+/* Queues a request (its candidates are copied; the caller's arrays may be
+ * reused at once).  PGX_E_CAPACITY if it can never fit the budget. */
+int pgx_serve_submit(pgx_server_t* s, const pgx_serve_request_t* req);
+/* Non-blocking: advances every width class whose iteration finished (retire,
+ * admit, launch) and returns up to max_out completed requests. */
+int pgx_serve_poll(pgx_server_t* s, pgx_serve_result_t* out, uint32_t max_out, uint32_t* n_out);
+int pgx_serve_stats(const pgx_server_t* s, pgx_serve_stats_t* st);
+int pgx_serve_destroy(pgx_server_t* s);
+const char* pgx_serve_last_error(const pgx_server_t* s);
 
 #ifdef __cplusplus
 }

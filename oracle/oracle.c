@@ -233,10 +233,12 @@ int orc_gen_multipliers(uint64_t seed, uint64_t m_lo, uint64_t m_hi, uint32_t d,
 
 /* survivors of X in [1, x_max] for each m (N = m*K + X with no prime factor
  * below sieve_limit), first per_m of each in ascending X. */
-int orc_gen_sieved(uint32_t P, uint32_t d, const uint64_t* m, uint32_t n_m, uint32_t x_max,
-                   uint32_t per_m, uint32_t sieve_limit, uint32_t S, uint32_t* out_limbs,
-                   uint32_t* out_m, uint32_t* out_x, size_t* written) {
-  if (!m || !out_limbs || !written || S == 0 || x_max == 0 || d == 0) return ORC_E_ARG;
+int orc_gen_sieved_side(uint32_t P, uint32_t d, int32_t sign, const uint64_t* m, uint32_t n_m,
+                        uint32_t x_max, uint32_t per_m, uint32_t sieve_limit, uint32_t S,
+                        uint32_t* out_limbs, uint32_t* out_m, uint32_t* out_x, size_t* written) {
+  if (!m || !out_limbs || !written || S == 0 || x_max == 0 || d == 0 ||
+      (sign != 1 && sign != -1))
+    return ORC_E_ARG;
   *written = 0;
   mpz_t K, mK, N;
   mpz_inits(K, mK, N, NULL);
@@ -265,15 +267,18 @@ int orc_gen_sieved(uint32_t P, uint32_t d, const uint64_t* m, uint32_t n_m, uint
     for (uint32_t k = 0; k < np; ++k) {
       const uint32_t p = primes[k];
       const uint64_t r = mpz_fdiv_ui(mK, p);
-      uint64_t x = (p - r) % p;
+      uint64_t x = (sign > 0) ? (p - r) % p : r; /* m*K + sign*x == 0 mod p */
       if (x == 0) x = p;
       for (; x <= x_max; x += p) comp[x] = 1;
     }
     uint32_t taken = 0;
     for (uint32_t x = 1; x <= x_max && taken < per_m; ++x) {
       if (comp[x]) continue;
-      mpz_add_ui(N, mK, x);
-      if (from_mpz(N, out_limbs + w * S, S) != ORC_OK) {
+      if (sign > 0)
+        mpz_add_ui(N, mK, x);
+      else
+        mpz_sub_ui(N, mK, x);
+      if (mpz_sgn(N) <= 0 || from_mpz(N, out_limbs + w * S, S) != ORC_OK) {
         rc = ORC_E_INPUT;
         break;
       }
@@ -288,6 +293,13 @@ int orc_gen_sieved(uint32_t P, uint32_t d, const uint64_t* m, uint32_t n_m, uint
   mpz_clears(K, mK, N, NULL);
   *written = w;
   return rc;
+}
+
+int orc_gen_sieved(uint32_t P, uint32_t d, const uint64_t* m, uint32_t n_m, uint32_t x_max,
+                   uint32_t per_m, uint32_t sieve_limit, uint32_t S, uint32_t* out_limbs,
+                   uint32_t* out_m, uint32_t* out_x, size_t* written) {
+  return orc_gen_sieved_side(P, d, 1, m, n_m, x_max, per_m, sieve_limit, S, out_limbs, out_m,
+                             out_x, written);
 }
 
 /* smallest prime factor of N below `limit`, or 0 (brute-force trial division) */

@@ -40,6 +40,10 @@ def _load():
                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                    ctypes.c_uint32, u32p, u32p, u32p,
                                    ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_gen_sieved_side.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, u64p,
+                                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, u32p,
+                                        ctypes.POINTER(ctypes.c_size_t)]
     lib.orc_small_factor.argtypes = [u32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
                                      u32p]
     lib.orc_time_powm2.argtypes = [u32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int,
@@ -96,15 +100,16 @@ def gen_multipliers(seed: int, m_lo: int, m_hi: int, d: int, n_m: int) -> np.nda
     return out
 
 
-def gen_sieved(P, d, m, x_max, per_m, limbs, sieve_limit=1_000_000):
+def gen_sieved(P, d, m, x_max, per_m, limbs, sieve_limit=1_000_000, sign=1):
+    """N = m*K + sign*X survivors (sign -1: the prev-prime side)."""
     m = np.ascontiguousarray(m, dtype=np.uint64)
     cap = len(m) * per_m
     out = np.zeros((cap, limbs), dtype=np.uint32)
     om = np.zeros(cap, dtype=np.uint32)
     ox = np.zeros(cap, dtype=np.uint32)
     w = ctypes.c_size_t(0)
-    rc = lib.orc_gen_sieved(P, d, _u64(m), len(m), x_max, per_m, sieve_limit, limbs, _u32(out),
-                            _u32(om), _u32(ox), ctypes.byref(w))
+    rc = lib.orc_gen_sieved_side(P, d, sign, _u64(m), len(m), x_max, per_m, sieve_limit, limbs,
+                                 _u32(out), _u32(om), _u32(ox), ctypes.byref(w))
     if rc != 0:
         raise ValueError(f"orc_gen_sieved failed: {rc}")
     n = w.value

@@ -70,6 +70,19 @@ uint32_t div_small(std::vector<uint32_t>& x, uint32_t d) {
   return (uint32_t)rem;
 }
 
+// x <- x - v; false if that would go negative.
+bool sub_small(std::vector<uint32_t>& x, uint32_t v) {
+  uint64_t borrow = v;
+  for (size_t j = 0; j < x.size() && borrow; ++j) {
+    const uint64_t cur = x[j];
+    x[j] = (uint32_t)(cur - borrow);
+    borrow = cur < borrow ? 1 : 0;
+  }
+  if (borrow) return false;
+  while (x.size() > 1 && x.back() == 0) x.pop_back();
+  return true;
+}
+
 uint32_t mod_small(const std::vector<uint32_t>& x, uint32_t p) {
   uint64_t rem = 0;
   for (size_t j = x.size(); j-- > 0;) rem = ((rem << 32) | x[j]) % p;
@@ -167,7 +180,15 @@ int pgx_gen_multipliers(uint64_t seed, uint64_t m_lo, uint64_t m_hi, uint32_t d,
 int pgx_gen_sieved(uint32_t P, uint32_t d, const uint64_t* m, uint32_t n_m, uint32_t x_max,
                    uint32_t per_m, uint32_t sieve_limit, uint32_t limbs, uint32_t* out_limbs,
                    uint32_t* out_m, uint32_t* out_x, size_t* written) {
-  if (!m || !out_limbs || !written || limbs == 0 || x_max == 0) return PGX_E_ARG;
+  return pgx_gen_sieved_side(P, d, +1, m, n_m, x_max, per_m, sieve_limit, limbs, out_limbs, out_m,
+                             out_x, written);
+}
+
+int pgx_gen_sieved_side(uint32_t P, uint32_t d, int32_t sign, const uint64_t* m, uint32_t n_m,
+                        uint32_t x_max, uint32_t per_m, uint32_t sieve_limit, uint32_t limbs,
+                        uint32_t* out_limbs, uint32_t* out_m, uint32_t* out_x, size_t* written) {
+  if (!m || !out_limbs || !written || limbs == 0 || x_max == 0 || (sign != 1 && sign != -1))
+    return PGX_E_ARG;
   *written = 0;
   std::vector<uint32_t> K;
   if (!primorial_over(P, d, K)) return PGX_E_INPUT;
@@ -181,7 +202,8 @@ int pgx_gen_sieved(uint32_t P, uint32_t d, const uint64_t* m, uint32_t n_m, uint
     for (size_t k = 0; k < primes.size(); ++k) {
       const uint32_t p = primes[k];
       const uint64_t r = (uint64_t)(m[mi] % p) * kmod[k] % p;  // m*K mod p
-      uint64_t x = (p - r) % p;                                // m*K + x == 0 mod p
+      // first x >= 1 with m*K + sign*x == 0 (mod p)
+      uint64_t x = (sign > 0) ? (p - r) % p : r;
       if (x == 0) x = p;
       for (; x <= x_max; x += p) comp[x] = 1;
     }
@@ -189,8 +211,13 @@ int pgx_gen_sieved(uint32_t P, uint32_t d, const uint64_t* m, uint32_t n_m, uint
     for (uint32_t x = 1; x <= x_max && taken < per_m; ++x) {
       if (comp[x]) continue;
       std::vector<uint32_t> N = K;
-      mul_small(N, m[mi], x);
-      if (N.size() > limbs || (bits_of(N) + 0) > 32u * limbs) return PGX_E_INPUT;
+      if (sign > 0) {
+        mul_small(N, m[mi], x);
+      } else {  // m*K - x (m*K > x_max for every configuration this builds)
+        mul_small(N, m[mi], 0);
+        if (!sub_small(N, x)) return PGX_E_INPUT;
+      }
+      if (N.size() > limbs || bits_of(N) > 32u * limbs) return PGX_E_INPUT;
       uint32_t* o = out_limbs + w * limbs;
       std::memset(o, 0, limbs * sizeof(uint32_t));
       std::memcpy(o, N.data(), N.size() * sizeof(uint32_t));

@@ -5,7 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "../../include/pgx.h"
+#include "../../include/pgx_defs.h"
 #include "prp.hpp"
 
 namespace pgx {

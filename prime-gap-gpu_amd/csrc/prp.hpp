@@ -20,7 +20,7 @@
 // only one copy of the unrolled squaring is emitted per base kind.
 #pragma once
 
-#include "../../include/pgx.h"
+#include "../../include/pgx_defs.h"
 #include "mont.hpp"
 
 namespace pgx {

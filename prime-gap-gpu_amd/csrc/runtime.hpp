@@ -61,6 +61,10 @@ hipError_t launch_fill(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_compact(const uint32_t* verdict, uint32_t n, uint32_t mask, uint32_t* index,
                           uint32_t* count, hipStream_t s);
 
+// dst[i] = src[index[i]] for i < n.
+hipError_t launch_gather(const uint32_t* src, const uint32_t* index, uint32_t n, uint32_t* dst,
+                         hipStream_t s);
+
 }  // namespace pgx
 
 #define PGX_HIP(h, call)                                                                   \

@@ -35,7 +35,21 @@ __global__ __launch_bounds__(256) void k_compact(const uint32_t* __restrict__ ve
   if (keep) index[base + rank] = i;
 }
 
+__global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ src,
+                                                const uint32_t* __restrict__ index, uint32_t n,
+                                                uint32_t* __restrict__ dst) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[index[i]];
+}
+
 }  // namespace
+
+hipError_t launch_gather(const uint32_t* src, const uint32_t* index, uint32_t n, uint32_t* dst,
+                         hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, src, index, n, dst);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
   const uint32_t blocks = n == 0 ? 1 : (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;

@@ -65,6 +65,7 @@ EXPORTED = (
     "pgx_last_error",
     "pgx_gen_sieved",
     "pgx_gen_multipliers",
+    "pgx_gen_sieved_side",
     "pgx_k_bits",
     "pgx_valu_rate",
     "pgx_test_persistent",
@@ -74,6 +75,12 @@ EXPORTED = (
     "pgx_plan_create",
     "pgx_plan_launch",
     "pgx_plan_destroy",
+    "pgx_serve_create",
+    "pgx_serve_submit",
+    "pgx_serve_poll",
+    "pgx_serve_stats",
+    "pgx_serve_destroy",
+    "pgx_serve_last_error",
 )
 
 
@@ -83,6 +90,31 @@ class Bucket(ctypes.Structure):
                 ("n", ctypes.c_uint32), ("max_bits", ctypes.c_uint32),
                 ("d_verdict", ctypes.c_void_p), ("d_residues", ctypes.c_void_p),
                 ("res_ld", ctypes.c_uint32)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("chunk", ctypes.c_uint32),
+                ("max_requests", ctypes.c_uint32), ("pool_columns", ctypes.c_uint32),
+                ("max_side", ctypes.c_uint32), ("stage_columns", ctypes.c_uint32)]
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("arrival_ns", ctypes.c_uint64),
+                ("limbs", ctypes.c_uint32), ("max_bits", ctypes.c_uint32),
+                ("n_next", ctypes.c_uint32), ("n_prev", ctypes.c_uint32),
+                ("next", ctypes.c_void_p), ("prev", ctypes.c_void_p)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("next_first", ctypes.c_int32),
+                ("prev_first", ctypes.c_int32), ("tests", ctypes.c_uint32),
+                ("latency_ns", ctypes.c_uint64)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("bytes_in_use", ctypes.c_uint64), ("tests", ctypes.c_uint64),
+                ("active", ctypes.c_uint32), ("waiting", ctypes.c_uint32),
+                ("in_flight", ctypes.c_uint32), ("finished_unread", ctypes.c_uint32)]
 
 
 class Tuning(ctypes.Structure):
@@ -121,6 +153,10 @@ def _load() -> ctypes.CDLL:
                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                    ctypes.c_uint32, u32p, u32p, u32p,
                                    ctypes.POINTER(ctypes.c_size_t)]
+    lib.pgx_gen_sieved_side.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, u64p,
+                                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, u32p,
+                                        ctypes.POINTER(ctypes.c_size_t)]
     lib.pgx_gen_multipliers.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                         ctypes.c_uint32, ctypes.c_uint32, u64p]
     lib.pgx_k_bits.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
@@ -136,6 +172,14 @@ def _load() -> ctypes.CDLL:
                                     ctypes.c_uint32]
     lib.pgx_plan_launch.argtypes = [vp, vp]
     lib.pgx_plan_destroy.argtypes = [vp]
+    lib.pgx_serve_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(ServeConfig)]
+    lib.pgx_serve_submit.argtypes = [vp, ctypes.POINTER(ServeRequest)]
+    lib.pgx_serve_poll.argtypes = [vp, ctypes.POINTER(ServeResult), ctypes.c_uint32,
+                                   ctypes.POINTER(ctypes.c_uint32)]
+    lib.pgx_serve_stats.argtypes = [vp, ctypes.POINTER(ServeStats)]
+    lib.pgx_serve_destroy.argtypes = [vp]
+    lib.pgx_serve_last_error.argtypes = [vp]
+    lib.pgx_serve_last_error.restype = ctypes.c_char_p
     return lib
 
 
@@ -196,20 +240,37 @@ def gen_multipliers(seed: int, m_lo: int, m_hi: int, d: int, n_m: int) -> np.nda
 
 
 def gen_sieved(P: int, d: int, m: np.ndarray, x_max: int, per_m: int, limbs: int,
-               sieve_limit: int = 1_000_000):
-    """Sieved candidates N = m*K + X: returns (limbs[count, limbs], m_index, X)."""
+               sieve_limit: int = 1_000_000, sign: int = 1):
+    """Sieved candidates N = m*K + sign*X: returns (limbs[count, limbs], m_index, X)."""
     m = np.ascontiguousarray(m, dtype=np.uint64)
     cap = len(m) * per_m
     out = np.zeros((cap, limbs), dtype=np.uint32)
     om = np.zeros(cap, dtype=np.uint32)
     ox = np.zeros(cap, dtype=np.uint32)
     w = ctypes.c_size_t(0)
-    rc = lib.pgx_gen_sieved(P, d, _u64(m), len(m), x_max, per_m, sieve_limit, limbs,
-                            _u32(out), _u32(om), _u32(ox), ctypes.byref(w))
+    rc = lib.pgx_gen_sieved_side(P, d, sign, _u64(m), len(m), x_max, per_m, sieve_limit, limbs,
+                                 _u32(out), _u32(om), _u32(ox), ctypes.byref(w))
     if rc != PGX_OK:
         raise PgxError(rc, "pgx_gen_sieved failed (d must divide P#, N must fit in limbs)")
     n = w.value
     return out[:n], om[:n], ox[:n]
+
+
+def gen_requests(P: int, d: int, n_req: int, seed: int, limbs: int, lo: int = 20, hi: int = 500,
+                 x_max: int = 30000, sieve_limit: int = 1_000_000):
+    """Next/prev-prime search requests for n_req multipliers m (coprime to d):
+    each side holds between lo and hi sieved candidates (uniform), ascending X;
+    next side N = m*K + X, prev side N = m*K - X.  Returns a list of
+    (m, next_limbs, prev_limbs)."""
+    m = gen_multipliers(seed, 10**7, 10**7 + 10**4, d, n_req)
+    nxt, mi_n, _ = gen_sieved(P, d, m, x_max, hi, limbs, sieve_limit, sign=1)
+    prv, mi_p, _ = gen_sieved(P, d, m, x_max, hi, limbs, sieve_limit, sign=-1)
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_req):
+        cn, cp = rng.integers(lo, hi + 1, size=2)
+        out.append((int(m[k]), nxt[mi_n == k][:cn], prv[mi_p == k][:cp]))
+    return out
 
 
 VALU_OPS = {0: "v_mad_u64_u32", 1: "v_mad_u64_u32_dependent", 2: "v_add_u32", 3: "v_mul_lo_u32",
@@ -399,3 +460,63 @@ class DeviceBatch:
 
     def residues(self) -> np.ndarray:
         return self.res.cpu().numpy().view(np.uint32)[:, :self.n].T.copy()
+
+
+class Server:
+    """pgx_server_t: next/prev-prime searches with continuous batching."""
+
+    def __init__(self, hbm_budget: int = 64 << 30, chunk: int = 32, max_requests: int = 8192,
+                 pool_columns: int = 1 << 21, max_side: int = 512, stage_columns: int = 1 << 17,
+                 device: int = 0):
+        cfg = ServeConfig(hbm_budget, chunk, max_requests, pool_columns, max_side, stage_columns)
+        h = ctypes.c_void_p()
+        rc = lib.pgx_serve_create(ctypes.byref(h), device, ctypes.byref(cfg))
+        if rc != PGX_OK:
+            raise PgxError(rc, lib.pgx_last_error(None).decode())
+        self._s = h
+        self._out = (ServeResult * 4096)()
+
+    def _check(self, rc):
+        if rc != PGX_OK:
+            raise PgxError(rc, lib.pgx_serve_last_error(self._s).decode())
+
+    def submit(self, rid: int, nxt: np.ndarray, prv: np.ndarray, max_bits: int,
+               arrival_ns: int = 0) -> None:
+        """nxt / prv: candidate-major limbs (same width), ascending X."""
+        nxt = np.ascontiguousarray(nxt, dtype=np.uint32)
+        prv = np.ascontiguousarray(prv, dtype=np.uint32)
+        limbs = nxt.shape[1] if nxt.size else prv.shape[1]
+        req = ServeRequest(rid, arrival_ns, limbs, max_bits, len(nxt), len(prv),
+                           nxt.ctypes.data if len(nxt) else None,
+                           prv.ctypes.data if len(prv) else None)
+        self._check(lib.pgx_serve_submit(self._s, ctypes.byref(req)))
+
+    def poll(self) -> list[tuple[int, int, int, int, int]]:
+        """(id, next_first, prev_first, tests, latency_ns) of completed requests."""
+        got = []
+        while True:
+            n = ctypes.c_uint32(0)
+            self._check(lib.pgx_serve_poll(self._s, self._out, len(self._out), ctypes.byref(n)))
+            got += [(r.id, r.next_first, r.prev_first, r.tests, r.latency_ns)
+                    for r in self._out[:n.value]]
+            if n.value < len(self._out):
+                return got
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        self._check(lib.pgx_serve_stats(self._s, ctypes.byref(st)))
+        return {k: int(getattr(st, k)) for k, _ in st._fields_}
+
+    def close(self):
+        if getattr(self, "_s", None):
+            lib.pgx_serve_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

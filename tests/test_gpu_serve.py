@@ -1,0 +1,113 @@
+"""GPU parity of the serving engine: next/prev-prime searches with continuous
+batching give the same first-PRP index per side alone and in the mix, equal to
+the GMP oracle's sequential search over the same sieved candidates."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pgx
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(337, 16), (503, 24), (907, 48)]  # d = 2310; 462 / 706 / 1263-bit candidates
+
+
+def say(msg):
+    print(f"[gpu-serve] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def first_prp(limbs):
+    if len(limbs) == 0:
+        return -1
+    v = O.verdicts(limbs, O.MODE_REFERENCE, (2,))
+    hit = np.nonzero(v == 1)[0]
+    return int(hit[0]) if len(hit) else -1
+
+
+@pytest.fixture(scope="module")
+def requests():
+    reqs = []
+    for P, words in SIZES:
+        for m, nxt, prv in pgx.gen_requests(P, 2310, 6, 0x5E0000 + P, words):
+            reqs.append((P, words, nxt, prv))
+    # one side with no PRP among few candidates is likely at 1263 bits; force
+    # a short one to exercise "none found"
+    P, words, nxt, prv = reqs[-1]
+    reqs.append((P, words, nxt[:3], prv[:2]))
+    return reqs
+
+
+@pytest.fixture(scope="module")
+def expected(requests):
+    return [(first_prp(n), first_prp(p)) for _, _, n, p in requests]
+
+
+def bits(P):
+    return pgx.k_bits(P, 2310) + 24 + 1  # m < 2^24
+
+
+def drain(srv, want, timeout=60.0):
+    got = {}
+    t0 = time.time()
+    while len(got) < want:
+        for rid, nf, pf, tests, lat in srv.poll():
+            got[rid] = (nf, pf, tests)
+        assert time.time() - t0 < timeout, srv.stats()
+    return got
+
+
+def test_alone_and_mixed_equal_oracle(requests, expected):
+    with pgx.Server(chunk=16) as srv:
+        alone = {}
+        for i, (P, words, nxt, prv) in enumerate(requests):
+            srv.submit(i, nxt, prv, bits(P))
+            alone.update(drain(srv, 1))
+        say(f"alone: {len(alone)} requests")
+        for i, (P, words, nxt, prv) in enumerate(requests):
+            srv.submit(1000 + i, nxt, prv, bits(P))
+        mixed = drain(srv, len(requests))
+        say(f"mixed: {len(mixed)} requests, stats {srv.stats()}")
+    for i, want in enumerate(expected):
+        assert alone[i][:2] == want, (i, alone[i], want)
+        assert mixed[1000 + i][:2] == want, (i, mixed[1000 + i], want)
+        # early exit: a side stops within one chunk after its first PRP
+        n, p = len(requests[i][2]), len(requests[i][3])
+        bound = sum(min(s, ((f // 16) + 1) * 16) if f >= 0 else s
+                    for f, s in zip(want, (n, p)))
+        assert alone[i][2] <= bound
+
+
+def test_budget_forces_queueing(requests, expected):
+    # room for about two 1263-bit requests at a time
+    budget = 2 * 1000 * (41 + 1) * 4
+    with pgx.Server(hbm_budget=budget, chunk=8, stage_columns=2048) as srv:
+        for i, (P, words, nxt, prv) in enumerate(requests):
+            srv.submit(i, nxt, prv, bits(P))
+        assert srv.stats()["waiting"] > 0
+        got = drain(srv, len(requests))
+        st = srv.stats()
+    assert st["bytes_in_use"] == 0 and st["active"] == 0
+    for i, want in enumerate(expected):
+        assert got[i][:2] == want
+
+
+def test_rejections_and_empty_request(requests):
+    P, words, nxt, prv = requests[0]
+    with pgx.Server(hbm_budget=1000, max_side=512) as srv:
+        with pytest.raises(pgx.PgxError) as e:  # needs more than the budget
+            srv.submit(1, nxt, prv, bits(P))
+        assert e.value.code == pgx.PGX_E_CAPACITY
+    with pgx.Server(max_side=16) as srv:
+        with pytest.raises(pgx.PgxError) as e:
+            srv.submit(2, nxt[:40], prv[:3], bits(P))
+        assert e.value.code == pgx.PGX_E_CAPACITY
+        with pytest.raises(pgx.PgxError) as e:  # candidates wider than max_bits says
+            srv.submit(3, nxt[:4], prv[:4], 300)
+        assert e.value.code == pgx.PGX_E_INPUT
+        srv.submit(4, nxt[:0], prv[:0], bits(P))
+        got = drain(srv, 1)
+        assert got[4][:2] == (-1, -1)

@@ -53,6 +53,20 @@ def test_generator_matches_oracle(P, d, limbs):
     assert len(a[0]) > 0
 
 
+@pytest.mark.parametrize("P,d,limbs", [(337, 2310, 16), (907, 2310, 48)])
+def test_prev_side_generator_matches_oracle(P, d, limbs):
+    m = pgx.gen_multipliers(17 + P, 10**7, 10**7 + 10**4, d, 2)
+    a = pgx.gen_sieved(P, d, m, 20000, 400, limbs, sign=-1)
+    b = O.gen_sieved(P, d, m, 20000, 400, limbs, sign=-1)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    # N = m*K - X, and no survivor has a factor below the sieve limit
+    # with no sieving primes, X = 1 survives: m*K + 1
+    mK = pgx.from_limbs(pgx.gen_sieved(P, d, m[:1], 1, 1, limbs, sieve_limit=2)[0])[0] - 1
+    assert pgx.from_limbs(a[0][:3]) == [mK - int(x) for x in a[2][:3]]
+    assert not O.small_factor(a[0][:50], 3000).any()
+
+
 def test_generator_rejects_bad_input():
     assert pgx.k_bits(337, 7 * 1009) == 0  # 1009 does not divide 337#
     with pytest.raises(pgx.PgxError):
