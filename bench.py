@@ -143,7 +143,9 @@ def bench_mixed(stream, peak: float, reps: int) -> dict:
 def bench_large(stream, peak: float, reps: int) -> dict:
     """P=907, d=30030 (~1260-bit N in 64-word buffers), 2^18 candidates:
     base-2 Fermat (persistent, autotuned), on-device compaction of the PRPs,
-    strong tests to bases 3 and 5 on the survivors only."""
+    strong tests to bases 3 and 5 on the survivors only.  Pipelined as four
+    sub-batches: Fermat + compaction on one stream, base 3 and base 5 each on
+    their own stream, so sub-batch k's Miller-Rabin overlaps k+1's Fermat."""
     import torch
 
     import oracle
@@ -151,49 +153,75 @@ def bench_large(stream, peak: float, reps: int) -> dict:
     L = gen(907, 30030, 64, 32, 8192, SEED + 907)
     n = len(L)
     a = pgx.DeviceBatch(L)
-    mr_verdict = torch.zeros(n, dtype=torch.int32, device="cuda")
-    mr_bucket = pgx.Bucket(a.cand.data_ptr(), a.ld, a.limbs, n, a.max_bits,
-                           mr_verdict.data_ptr(), 0, a.ld)
+    v3 = torch.zeros(n, dtype=torch.int32, device="cuda")
+    v5 = torch.zeros(n, dtype=torch.int32, device="cuda")
     index = torch.zeros(n, dtype=torch.int32, device="cuda")
-    count = torch.zeros(1, dtype=torch.int32, device="cuda")
+    count = torch.zeros(64, dtype=torch.int32, device="cuda")
     fermat = pgx.Runner(1, limbs=64)
-    extra = pgx.Runner(1, limbs=64, bases=[3, 5])
+    r3 = pgx.Runner(1, limbs=64, bases=[3])
+    r5 = pgx.Runner(1, limbs=64, bases=[5])
     t, rate = fermat.autotune(a.bucket(), pgx.MODE_FERMAT)
     log(f"large: tuned {t.as_dict()} {rate / 1e6:.3f} M/s")
+    s3, s5 = torch.cuda.Stream(), torch.cuda.Stream()
+    SUB = 4
+    step = n // SUB
+
+    def sl(ptr, k):  # column offset k * step of a limb-interleaved buffer
+        return ptr + 4 * k * step
 
     def fermat_only():
         fermat.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True,
                                stream=stream.cuda_stream)
 
     def pipeline():
-        fermat_only()
-        fermat.compact(a.verdict.data_ptr(), n, pgx.FLAG_FERMAT, index.data_ptr(),
-                       count.data_ptr(), stream.cuda_stream)
-        extra.test_indexed(mr_bucket, index.data_ptr(), count.data_ptr(), pgx.MODE_STRONG,
-                           stream=stream.cuda_stream)
+        for k in range(SUB):
+            fb = pgx.Bucket(sl(a.cand.data_ptr(), k), a.ld, a.limbs, step, a.max_bits,
+                            sl(a.verdict.data_ptr(), k), sl(a.res.data_ptr(), k), a.ld)
+            fermat.test_persistent(fb, pgx.MODE_FERMAT, residues=True, stream=stream.cuda_stream)
+            fermat.compact(fb.d_verdict, step, pgx.FLAG_FERMAT, sl(index.data_ptr(), k),
+                           count.data_ptr() + 4 * k, stream.cuda_stream)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            for run, st, vv in ((r3, s3, v3), (r5, s5, v5)):
+                st.wait_event(ev)
+                mb = pgx.Bucket(fb.d_cand, a.ld, a.limbs, step, a.max_bits,
+                                sl(vv.data_ptr(), k), 0, a.ld)
+                run.test_indexed(mb, sl(index.data_ptr(), k), count.data_ptr() + 4 * k,
+                                 pgx.MODE_STRONG, stream=st.cuda_stream)
+        for st in (s3, s5):  # join the side streams back for the timing events
+            ev = torch.cuda.Event()
+            ev.record(st)
+            stream.wait_event(ev)
+
     pipeline()
     ms_fermat = time_ms(stream, fermat_only, reps)
+    v3.fill_(-1)
+    v5.fill_(-1)
     ms_all = time_ms(stream, pipeline, reps)
-    k = int(count.item())
+    k = int(count[:SUB].sum().item())
     v = a.verdicts()
-    surv = np.sort(index.cpu().numpy()[:k])
-    mv = mr_verdict.cpu().numpy().view(np.uint32)
+    f3 = v3.cpu().numpy().view(np.uint32)
+    f5 = v5.cpu().numpy().view(np.uint32)
+    surv = np.nonzero(v & 1)[0]
+    prp = surv[(f3[surv] == 1) & (f5[surv] == 1)]
     ov = oracle.verdicts(L[:1024], oracle.MODE_FERMAT)
     s_sample = surv[:256]
-    ok = bool(np.array_equal(v[:1024], ov) and
-              np.array_equal(mv[s_sample], oracle.verdicts(L[s_sample], oracle.MODE_STRONG, (3, 5)))
-              and k == int((v & 1).sum()))
+    want = oracle.verdicts(L[s_sample], oracle.MODE_STRONG, (3, 5))
+    ok = bool(np.array_equal(v[:1024], ov) and k == len(surv) and
+              np.array_equal((f3[s_sample] == 1).astype(np.uint32) +
+                             ((f3[s_sample] == 1) & (f5[s_sample] == 1)).astype(np.uint32), want))
     bits = median_bits(L)
-    fermat.close()
-    extra.close()
+    for r in (fermat, r3, r5):
+        r.close()
     return {
         "workload": "P=907 d=30030, 2^18 sieved candidates (~1260 bits) in 64-word buffers: "
-                    "Fermat base 2 -> device compaction -> strong bases 3,5 on survivors",
+                    "Fermat base 2 -> device compaction -> strong bases 3 and 5 on survivors, "
+                    "4 sub-batches, Miller-Rabin on two side streams",
         "candidates": n, "bits": bits, "kernel_limbs": kernel_limbs(a.max_bits),
         "tuned": t.as_dict(),
         "fermat_ms": ms_fermat, "fermat_tests_per_s": n / (ms_fermat * 1e-3),
         "pipeline_ms": ms_all, "pipeline_tests_per_s": n / (ms_all * 1e-3),
-        "survivors": k,
+        "survivors": k, "prp_all_bases": int(len(prp)),
         "fermat_valu_frac": work_per_test(bits) * n / (ms_fermat * 1e-3) / peak,
         "parity_sample_ok": ok,
     }
@@ -306,6 +334,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline plain-kernel workload")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (a kernel of known "
+                         "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path "
@@ -383,6 +414,13 @@ def main() -> int:
     # measured v_mad_u64_u32 issue rate of this device
     macs_launch = work_per_test(L) * BATCH
     achieved = macs_launch / (avg_launch_ms * 1e-3)
+    if args.copy_probe:  # 256 MiB read + 256 MiB written by one copy kernel
+        src = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        with torch.cuda.stream(stream):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     rate = {name: pgx.valu_rate(op) for op, name in pgx.VALU_OPS.items()}
     peak = rate["v_mad_u64_u32"]
     log("VALU rates (lane-ops/s): " + ", ".join(f"{k}={v:.3e}" for k, v in rate.items() if v))

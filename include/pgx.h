@@ -81,6 +81,81 @@ int pgx_test_device(pgx_runner_t* h, const uint32_t* d_cand, uint32_t ld, uint32
                     size_t n, uint32_t max_bits, uint32_t flags, uint32_t* d_verdict,
                     uint32_t* d_residues, uint32_t res_ld, void* stream);
 
+/* fill_batch on the device (gpu_testing.cu:136-195 builds N = m*K + x with
+ * mpz_mul_ui + mpz_add_ui per candidate, then run_test exports the limbs).
+ * pgx_submit_mk uploads K (k_words little-endian words; cached while the same
+ * pointer and size come back) and n multipliers m[i], builds
+ * N_i = m[i]*K + sign*x on the device in the runner's limb buffers and launches
+ * the test; pgx_wait returns the verdicts as for pgx_submit, or PGX_E_INPUT when
+ * a candidate needed more than the runner's `limbs` words or was negative.  An
+ * even candidate or one <= 1 (which a sieve never passes) gets
+ * PGX_VERDICT_INVALID. */
+int pgx_submit_mk(pgx_runner_t* h, const uint32_t* K, uint32_t k_words, const uint64_t* m,
+                  size_t n, uint32_t x, int32_t sign, uint32_t flags);
+
+/* The construction kernel alone, on device buffers: column i of d_out
+ * (out_words rows, leading dimension ld) = d_m[i]*K + sign*(d_x ? d_x[i] : x0).
+ * *d_status (device word, caller-cleared) becomes nonzero when a candidate
+ * does not fit out_words words or is negative. */
+int pgx_build_candidates(pgx_runner_t* h, const uint32_t* d_K, uint32_t k_words,
+                         const uint64_t* d_m, const uint32_t* d_x, uint32_t x0, int32_t sign,
+                         uint32_t n, uint32_t* d_out, uint32_t ld, uint32_t out_words,
+                         uint32_t* d_status, void* stream);
+
+/* ---- m-interval search: sieve, test and bookkeeping on the device ------------
+ * One object per interval of multipliers m = m_start + m_i (m_i odd, m_i <
+ * m_inc), driven one X at a time like the reference's sieve thread and GPU
+ * thread (gap_search_gpu.cpp:462-804, gpu_testing.cu:104-354).  Every per-m set
+ * is a bitmap in HBM, bit t <-> m_i = 2t + 1, m_inc/64 + 2 words:
+ *   active      odd m_i with gcd(m_start + m_i, d) = 1 not yet found prime
+ *               (is_coprime_and_valid_m :263-304, remove_vector :340-351);
+ *   composites  m_i whose m*K + X has an odd factor q < max_prime with q > P or
+ *               q | d, for the last sieved X (run_sieve_thread :577-689);
+ *   found       m_i with a probable prime m*K + X at some tested X
+ *               (TestData::found_prime_m_i, process_finished_batch).
+ * The runner supplies the device, stream, bases (rounds) and the batch buffers:
+ * a step borrows them, so the runner takes no other batch meanwhile. */
+typedef struct pgx_search pgx_search_t;
+
+#define PGX_SEARCH_AS_WRITTEN 1u /* primes >= 100000 use the reference's
+                                    arithmetic as written (:651-689): a 32-bit
+                                    X*neg_inv_K, and an un-halved start when
+                                    X*max_prime >= m_start */
+#define PGX_SEARCH_ACTIVE 0u
+#define PGX_SEARCH_COMPOSITES 1u
+#define PGX_SEARCH_FOUND 2u
+
+typedef struct {
+  uint32_t P;         /* K = P# / d */
+  uint32_t d;         /* even */
+  uint64_t m_start;   /* even */
+  uint32_t m_inc;     /* even, < 2^31 */
+  uint32_t max_prime; /* sieve limit (exclusive), 1000 .. 2e9 */
+  uint32_t flags;     /* PGX_SEARCH_AS_WRITTEN */
+} pgx_search_config_t;
+
+/* K: k_words little-endian words, odd.  PGX_E_INPUT when (m_start+m_inc)*K
+ * does not fit the runner's words. */
+int pgx_search_create(pgx_search_t** out, pgx_runner_t* runner, const uint32_t* K,
+                      uint32_t k_words, const pgx_search_config_t* cfg);
+int pgx_search_destroy(pgx_search_t* s);
+/* Sieve the interval at X (even) and list the tests: active m_i that are
+ * neither composite at X nor found, ascending (the reference's next_sieves
+ * entry for X).  *n_tests = their count. */
+int pgx_search_sieve(pgx_search_t* s, uint32_t X, uint32_t* n_tests);
+/* Test every listed m_i: N = (m_start + m_i) K + X with the runner's bases
+ * (every base must pass, miller_rabin.h:405-407); passing m_i join found and
+ * leave active.  *n_found = how many passed at this X. */
+int pgx_search_test(pgx_search_t* s, uint32_t* n_found);
+int pgx_search_step(pgx_search_t* s, uint32_t X, uint32_t* n_tests, uint32_t* n_found);
+/* The tests list of the last sieved X (cap entries; *n = its length). */
+int pgx_search_tests(pgx_search_t* s, uint32_t* out, size_t cap, size_t* n);
+/* Every (m_i, X) found so far, ascending in m_i. */
+int pgx_search_found(pgx_search_t* s, uint32_t* m_i, uint32_t* x, size_t cap, size_t* n);
+/* A copy of one bitmap (PGX_SEARCH_ACTIVE / _COMPOSITES / _FOUND). */
+int pgx_search_bitmap(pgx_search_t* s, uint32_t which, uint32_t* out, size_t cap, size_t* n);
+const char* pgx_search_last_error(pgx_search_t* s);
+
 /* Message of the last failing call on h (or of the last failing create when
  * h is NULL).  Never NULL. */
 const char* pgx_last_error(const pgx_runner_t* h);
@@ -200,7 +275,7 @@ int pgx_valu_rate(int device, int op, uint32_t blocks, uint32_t iters, double* p
  * counterpart exists in the reference (which has no serving path). */
 typedef struct {
   uint64_t hbm_budget;   /* bytes of admitted candidate + verdict storage */
-  uint32_t chunk;        /* candidates per side per iteration */
+  uint32_t chunk;        /* candidates per side per iteration at 512 bits (scaled by width) */
   uint32_t max_requests; /* admitted requests per width class */
   uint32_t pool_columns; /* candidate columns per width class pool */
   uint32_t max_side;     /* most candidates one side may carry */

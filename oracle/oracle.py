@@ -11,6 +11,9 @@ Reference functions followed (file:line in sethtroisi/prime-gap-gpu):
   residue          BASELINE CPU path: mpz_powm(2, N-1, N)
   gen_*            gap_common.cpp:60-65 (K = P#/d), gpu_testing.cu:171-172 (N = m*K + x),
                    gap_search_gpu.cpp:619-689 (sieve of m*K + X by small primes)
+  ms_*             the m-interval search step (msieve.c): is_coprime_and_valid_m,
+                   coprime_X, run_sieve_thread's composites bitmap, finalize +
+                   fill_batch + process_finished_batch + remove_vector
 """
 from __future__ import annotations
 
@@ -49,6 +52,15 @@ def _load():
     lib.orc_time_powm2.argtypes = [u32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int,
                                    ctypes.c_int, u32p]
     lib.orc_time_powm2.restype = ctypes.c_double
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    lib.orc_ms_active.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                  ctypes.c_uint32, u32p, szp]
+    lib.orc_ms_coprime_x.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_size_t, szp]
+    lib.orc_ms_sieve.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
+                                 ctypes.c_uint32, ctypes.c_int, u32p]
+    lib.orc_ms_step.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, u32p,
+                                u32p, szp, u32p, u32p, ctypes.c_uint32, u32p, szp, u32p, szp]
     return lib
 
 
@@ -129,6 +141,62 @@ def time_powm2(limbs: np.ndarray, threads: int, warmup: bool = True):
     cnt = np.zeros(1, dtype=np.uint32)
     sec = lib.orc_time_powm2(_u32(a), a.shape[0], a.shape[1], threads, int(warmup), _u32(cnt))
     return sec, int(cnt[0])
+
+
+# ---- m-interval search step (msieve.c) ------------------------------------
+
+def bitmap_words(m_inc: int) -> int:
+    return m_inc // 64 + 2
+
+
+def ms_active(P: int, d: int, m_start: int, m_inc: int) -> np.ndarray:
+    n = ctypes.c_size_t(0)
+    rc = lib.orc_ms_active(P, d, m_start, m_inc, None, ctypes.byref(n))
+    if rc != 0:
+        raise ValueError(f"orc_ms_active failed: {rc}")
+    out = np.zeros(max(1, n.value), dtype=np.uint32)
+    lib.orc_ms_active(P, d, m_start, m_inc, _u32(out), ctypes.byref(n))
+    return out[:n.value]
+
+
+def ms_coprime_x(P: int, d: int) -> np.ndarray:
+    n = ctypes.c_size_t(0)
+    lib.orc_ms_coprime_x(P, d, None, 0, ctypes.byref(n))
+    out = np.zeros(max(1, n.value), dtype=np.uint32)
+    lib.orc_ms_coprime_x(P, d, _u32(out), n.value, ctypes.byref(n))
+    return out[:n.value]
+
+
+def ms_sieve(K_words: np.ndarray, P: int, d: int, m_start: int, m_inc: int, max_prime: int,
+             X: int, as_written: bool = False) -> np.ndarray:
+    """Composites bitmap (bit t <-> m_i = 2t + 1), m_inc/64 + 2 words."""
+    k = np.ascontiguousarray(K_words, dtype=np.uint32)
+    out = np.zeros(bitmap_words(m_inc), dtype=np.uint32)
+    rc = lib.orc_ms_sieve(_u32(k), len(k), P, d, m_start, m_inc, max_prime, X, int(as_written),
+                          _u32(out))
+    if rc != 0:
+        raise ValueError(f"orc_ms_sieve failed: {rc}")
+    return out
+
+
+def ms_step(K_words, m_start: int, X: int, composites, active: np.ndarray, found: np.ndarray,
+            bases=(2,)):
+    """One X step; returns (tests, found_mi) and updates active (returned
+    trimmed) and found (in place)."""
+    k = np.ascontiguousarray(K_words, dtype=np.uint32)
+    act = np.ascontiguousarray(active, dtype=np.uint32).copy()
+    na = ctypes.c_size_t(len(act))
+    b = np.asarray(bases, dtype=np.uint32)
+    tests = np.zeros(max(1, len(act)), dtype=np.uint32)
+    fmi = np.zeros(max(1, len(act)), dtype=np.uint32)
+    nt, nf = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    assert found.dtype == np.uint32 and found.flags.c_contiguous
+    rc = lib.orc_ms_step(_u32(k), len(k), m_start, X, _u32(np.ascontiguousarray(composites)),
+                         _u32(act), ctypes.byref(na), _u32(found), _u32(b), len(b), _u32(tests),
+                         ctypes.byref(nt), _u32(fmi), ctypes.byref(nf))
+    if rc != 0:
+        raise ValueError(f"orc_ms_step failed: {rc}")
+    return tests[:nt.value], fmi[:nf.value], act[:na.value]
 
 
 # ---- pure-Python restatements (CPython bignum; small inputs) ---------------

@@ -17,7 +17,10 @@
 // Montgomery square or product x < 2N; after a doubling x < 4N; and a
 // Montgomery square of any x < 4N returns < (16N^2 + RN)/R < 2N.  So the base-2
 // doubling is a bare shift by one bit and no conditional subtraction appears
-// in the hot loop.
+// in the hot loop.  Where the column bound leaves room (RB = 28, or RB = 29
+// with S <= 21) the doubling is folded into the squaring itself
+// (mont_sqr_dbl: (2^bit x^2 + mN) / R), which removes the shift pass and its
+// dependency chain from the loop.
 //
 // The squaring is finely-integrated product scanning: column k of x^2 (cross
 // products accumulated once and doubled) and of m*N share one accumulator, the
@@ -85,6 +88,47 @@ PGX_FN void mont_sqr(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv) {
       else t0 += mul64(m[i], n[k - i]);
     }
     uint64_t t = t0 + t1 + ((c0 + c1) << 1);
+    if (k < S) {
+      m[k] = ((uint32_t)t * ninv) & MASK;
+      t += mul64(m[k], n[0]);  // low RB bits become zero
+    } else {
+      x[k - S] = (uint32_t)t & MASK;
+    }
+    carry = t >> RB;
+  }
+  x[S - 1] = (uint32_t)carry;
+}
+
+// x <- 2^bit * x^2 / R mod N with bit in {0, 1}: the base-2 step of a bit of
+// the exponent folded into the squaring (cross products shifted by 1 + bit,
+// the square term by bit), so no separate doubling pass.  x < 2N in and out:
+// (2 * 4N^2 + R N) / R < 2N because R >= 8N.  Column sums stay below 2^64 for
+// RB = 28 (any S <= 80) and for RB = 29 up to S = 21 (1.5 S 2^59 < 2^64).
+template <int S>
+PGX_FN void mont_sqr_dbl(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv, uint32_t bit) {
+  constexpr int RB = Radix<S>::RB;
+  constexpr uint32_t MASK = Radix<S>::MASK;
+  static_assert(RB == 28 || S <= 21, "column bound of the folded doubling");
+  uint32_t m[S];
+  uint64_t carry = 0;
+#pragma unroll
+  for (int k = 0; k < 2 * S - 1; ++k) {
+    const int i0 = (k < S) ? 0 : k - S + 1;
+    uint64_t c0 = 0, c1 = 0;
+#pragma unroll
+    for (int i = i0; i < k - i; ++i) {
+      if ((i - i0) & 1) c1 += mul64(x[i], x[k - i]);
+      else c0 += mul64(x[i], x[k - i]);
+    }
+    uint64_t t0 = carry, t1 = 0;
+    const int i1 = (k < S) ? k : S;
+#pragma unroll
+    for (int i = i0; i < i1; ++i) {
+      if ((i - i0) & 1) t1 += mul64(m[i], n[k - i]);
+      else t0 += mul64(m[i], n[k - i]);
+    }
+    uint64_t t = t0 + t1 + ((c0 + c1) << (1 + bit));
+    if ((k & 1) == 0) t += mul64(x[k >> 1], x[k >> 1]) << bit;
     if (k < S) {
       m[k] = ((uint32_t)t * ninv) & MASK;
       t += mul64(m[k], n[0]);  // low RB bits become zero

@@ -65,11 +65,18 @@ void release(pgx_runner* h) {
   if (h->d_verdict) (void)hipFree(h->d_verdict);
   if (h->d_res) (void)hipFree(h->d_res);
   if (h->d_counters) (void)hipFree(h->d_counters);
+  if (h->d_K) (void)hipFree(h->d_K);
+  if (h->d_m) (void)hipFree(h->d_m);
+  if (h->d_status) (void)hipFree(h->d_status);
+  if (h->h_m) (void)hipHostFree(h->h_m);
+  if (h->h_status) (void)hipHostFree(h->h_status);
 }
 
-int launch(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs, size_t n,
-           uint32_t max_bits, uint32_t flags, uint32_t* d_verdict, uint32_t* d_res,
-           uint32_t res_ld, hipStream_t stream) {
+}  // namespace
+
+int pgx::launch_prp(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs, size_t n,
+                    uint32_t max_bits, uint32_t flags, uint32_t* d_verdict, uint32_t* d_res,
+                    uint32_t res_ld, hipStream_t stream) {
   const int S = pgx::kernel_for_bits(max_bits);
   if (S == 0)
     return fail(h, PGX_E_INPUT, "candidates of %u bits exceed the widest kernel (%d bits)",
@@ -89,7 +96,7 @@ int launch(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs, s
   return PGX_OK;
 }
 
-}  // namespace
+using pgx::launch_prp;
 
 extern "C" {
 
@@ -184,7 +191,7 @@ int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t fla
   PGX_HIP(h, hipSetDevice(h->device));
   PGX_HIP(h, hipMemcpyAsync(h->d_cand, h->h_cand, (size_t)S * n * 4, hipMemcpyHostToDevice,
                             h->stream));
-  int rc = launch(h, h->d_cand, (uint32_t)n, S, n, max_bits, flags, h->d_verdict, h->d_res,
+  int rc = launch_prp(h, h->d_cand, (uint32_t)n, S, n, max_bits, flags, h->d_verdict, h->d_res,
                   (uint32_t)n, h->stream);
   if (rc != PGX_OK) return rc;
   PGX_HIP(h, hipMemcpyAsync(h->h_verdict, h->d_verdict, n * 4, hipMemcpyDeviceToHost, h->stream));
@@ -197,6 +204,64 @@ int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t fla
   return PGX_OK;
 }
 
+int pgx_submit_mk(pgx_runner_t* h, const uint32_t* K, uint32_t k_words, const uint64_t* m,
+                  size_t n, uint32_t x, int32_t sign, uint32_t flags) {
+  if (!h) return PGX_E_ARG;
+  if (h->pending) return fail(h, PGX_E_STATE, "a batch is already in flight");
+  if (!K || k_words == 0 || (n && !m) || (sign != 1 && sign != -1))
+    return fail(h, PGX_E_ARG, "bad pgx_submit_mk arguments");
+  if (n > h->max_n) return fail(h, PGX_E_CAPACITY, "tests(%zu) > n(%zu)", n, h->max_n);
+  if ((flags & PGX_MODE_MASK) > PGX_MODE_FERMAT) return fail(h, PGX_E_ARG, "bad mode");
+  if (n == 0) {
+    h->pending = true;
+    h->pending_n = 0;
+    h->pending_flags = flags;
+    return PGX_OK;
+  }
+  // Width bound: bits(m*K + x) <= bits(K) + bits(max m) + 1.
+  const uint32_t kb = bit_length(K, k_words);
+  uint64_t mmax = 0;
+  for (size_t i = 0; i < n; ++i) mmax = std::max(mmax, m[i]);
+  const uint32_t mb = mmax ? 64u - __builtin_clzll(mmax) : 0u;
+  const uint32_t bound = std::max<uint32_t>(kb + mb + 1, 2u);
+  const uint32_t max_bits = std::min<uint32_t>(bound, 32u * h->limbs);
+  PGX_HIP(h, hipSetDevice(h->device));
+  if (!h->d_m) {
+    PGX_HIP(h, hipMalloc((void**)&h->d_m, h->max_n * 8));
+    PGX_HIP(h, hipHostMalloc((void**)&h->h_m, h->max_n * 8, hipHostMallocDefault));
+    PGX_HIP(h, hipMalloc((void**)&h->d_status, 4));
+    PGX_HIP(h, hipHostMalloc((void**)&h->h_status, 4, hipHostMallocDefault));
+  }
+  if (h->h_K.size() != k_words || std::memcmp(h->h_K.data(), K, (size_t)k_words * 4) != 0) {
+    // K changed: upload it once (the stream is idle, no batch is in flight)
+    if (h->d_K) (void)hipFree(h->d_K);
+    h->d_K = nullptr;
+    h->h_K.clear();
+    PGX_HIP(h, hipMalloc((void**)&h->d_K, (size_t)k_words * 4));
+    PGX_HIP(h, hipMemcpy(h->d_K, K, (size_t)k_words * 4, hipMemcpyHostToDevice));
+    h->h_K.assign(K, K + k_words);
+  }
+  std::memcpy(h->h_m, m, n * 8);
+  PGX_HIP(h, hipMemcpyAsync(h->d_m, h->h_m, n * 8, hipMemcpyHostToDevice, h->stream));
+  PGX_HIP(h, pgx::launch_fill(h->d_status, 1, 0u, h->stream));
+  int rc = pgx_build_candidates(h, h->d_K, k_words, h->d_m, nullptr, x, sign, (uint32_t)n,
+                                h->d_cand, (uint32_t)n, h->limbs, h->d_status, h->stream);
+  if (rc != PGX_OK) return rc;
+  rc = launch_prp(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, h->d_verdict, h->d_res,
+              (uint32_t)n, h->stream);
+  if (rc != PGX_OK) return rc;
+  PGX_HIP(h, hipMemcpyAsync(h->h_status, h->d_status, 4, hipMemcpyDeviceToHost, h->stream));
+  PGX_HIP(h, hipMemcpyAsync(h->h_verdict, h->d_verdict, n * 4, hipMemcpyDeviceToHost, h->stream));
+  if (flags & PGX_WANT_RESIDUE)
+    PGX_HIP(h, hipMemcpyAsync(h->h_res, h->d_res, (size_t)h->limbs * n * 4, hipMemcpyDeviceToHost,
+                              h->stream));
+  h->pending = true;
+  h->pending_n = n;
+  h->pending_flags = flags;
+  h->status_pending = true;
+  return PGX_OK;
+}
+
 int pgx_wait(pgx_runner_t* h, uint32_t* verdict, uint32_t* residues) {
   if (!h) return PGX_E_ARG;
   if (!h->pending) return fail(h, PGX_E_STATE, "no batch in flight");
@@ -204,6 +269,10 @@ int pgx_wait(pgx_runner_t* h, uint32_t* verdict, uint32_t* residues) {
   const size_t n = h->pending_n;
   if (n == 0) return PGX_OK;
   PGX_HIP(h, hipStreamSynchronize(h->stream));
+  if (h->status_pending) {
+    h->status_pending = false;
+    if (*h->h_status) return fail(h, PGX_E_INPUT, "a candidate m*K +- x did not fit or was negative");
+  }
   if (verdict) std::memcpy(verdict, h->h_verdict, n * 4);
   if ((h->pending_flags & PGX_WANT_RESIDUE) && residues) {
     const uint32_t S = h->limbs;
@@ -234,7 +303,7 @@ int pgx_test_device(pgx_runner_t* h, const uint32_t* d_cand, uint32_t ld, uint32
   if (max_bits < 2) return fail(h, PGX_E_ARG, "max_bits must be >= 2");
   PGX_HIP(h, hipSetDevice(h->device));
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-  return launch(h, d_cand, ld, limbs, n, max_bits, flags, d_verdict, d_residues, res_ld, s);
+  return launch_prp(h, d_cand, ld, limbs, n, max_bits, flags, d_verdict, d_residues, res_ld, s);
 }
 
 const char* pgx_last_error(const pgx_runner_t* h) {

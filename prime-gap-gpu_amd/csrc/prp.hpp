@@ -11,9 +11,10 @@
 // computed anyway and the tests hold them equal.)
 // The exponentiation is left-to-right binary over the bits of N-1 (the
 // reference uses a fixed 2^W window table, miller_rabin.h:160-214; the result
-// does not depend on the chunking).  For base 2 the per-bit multiply is a
-// one-bit shift (lazy doubling), so base 2 needs no table and no second
-// Montgomery product.
+// does not depend on the chunking).  For base 2 the per-bit multiply by 2 is
+// folded into the squaring ((2^bit x^2 + mN) / R, mont_sqr_dbl) where the
+// column bound allows it, and is a one-bit shift otherwise, so base 2 needs
+// no table and no second Montgomery product.
 //
 // One loop covers all bits L-2 .. 0 of N-1: the bits below t are zero, so the
 // strong-test checks of y_0..y_{t-1} ride on the same squaring call site and
@@ -36,9 +37,11 @@ struct Bases {
   uint32_t a[PGX_MAX_BASES];
 };
 
-// Base 2: multiply by the base = lazy doubling.
+// Base 2: multiply by the base = doubling, folded into the squaring when the
+// radix leaves room for it (RB = 28, or RB = 29 with S <= 21).
 template <int S>
 struct MulBy2 {
+  static constexpr bool kFolded = (Radix<S>::RB == 28) || (S <= 21);
   PGX_FN void operator()(uint32_t (&x)[S], const uint32_t (&)[S], uint32_t, uint32_t bit) const {
     dbl_lazy<S>(x, bit);
   }
@@ -47,6 +50,7 @@ struct MulBy2 {
 // General base: Montgomery product by am = Mont(a) on set bits.
 template <int S>
 struct MulByMont {
+  static constexpr bool kFolded = false;
   const uint32_t (&am)[S];
   PGX_FN void operator()(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv,
                          uint32_t bit) const {
@@ -71,9 +75,14 @@ PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t 
   uint32_t w = 0;
   for (int p = L - 2; p >= 0; --p) {
     if (p == L - 2 || (p & 31) == 31) w = word(p >> 5);
-    mont_sqr<S>(x, n, ninv);
     // bit p of N-1: bit p of N for p >= 1, zero for p == 0
-    step(x, n, ninv, (p > 0) ? ((w >> (p & 31)) & 1u) : 0u);
+    const uint32_t bit = (p > 0) ? ((w >> (p & 31)) & 1u) : 0u;
+    if constexpr (Step::kFolded) {
+      mont_sqr_dbl<S>(x, n, ninv, bit);
+    } else {
+      mont_sqr<S>(x, n, ninv);
+      step(x, n, ninv, bit);
+    }
     if (p <= t && p >= 1) {  // x = Mont(y_{t-p})
       if (p == t) r_mod_n<S>(one, n, L);
       canon<S>(x, n);

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <map>
 #include <string>
+#include <vector>
 
 #include "../../include/pgx.h"
 #include "launch.hpp"
@@ -27,6 +28,14 @@ struct pgx_runner {
   uint32_t* d_verdict = nullptr;
   uint32_t* d_res = nullptr;
   uint32_t* d_counters = nullptr;  // work counters of persistent launches
+  // device construction (pgx_submit_mk): K cache, multipliers, status word
+  std::vector<uint32_t> h_K;  // the K held in d_K
+  uint32_t* d_K = nullptr;
+  uint64_t* h_m = nullptr;  // pinned
+  uint64_t* d_m = nullptr;
+  uint32_t* d_status = nullptr;
+  uint32_t* h_status = nullptr;  // pinned
+  bool status_pending = false;   // the pending batch was built on the device
   // pending batch
   bool pending = false;
   size_t pending_n = 0;
@@ -60,6 +69,19 @@ hipError_t launch_fill(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 // Appends columns with verdict != INVALID && (verdict & mask) to index.
 hipError_t launch_compact(const uint32_t* verdict, uint32_t n, uint32_t mask, uint32_t* index,
                           uint32_t* count, hipStream_t s);
+
+// Plain PRP launch (one lane per column) with the runner's bases; PGX_E_*.
+int launch_prp(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs, size_t n,
+               uint32_t max_bits, uint32_t flags, uint32_t* d_verdict, uint32_t* d_res,
+               uint32_t res_ld, hipStream_t stream);
+
+// Column i of out (out_words rows, leading dimension ld) = m_i * K + sign * x_i
+// with m_i = m64[i] (m64 != NULL) or m_base + m_off[i], and x_i = x[i] (x != NULL)
+// or x0.  *status |= 1 when a column does not fit out_words or is negative.
+hipError_t launch_build(const uint32_t* K, uint32_t k_words, const uint64_t* m64,
+                        const uint32_t* m_off, uint64_t m_base, const uint32_t* x, uint32_t x0,
+                        int32_t sign, uint32_t n, uint32_t* out, uint32_t ld, uint32_t out_words,
+                        uint32_t* status, hipStream_t s);
 
 // dst[i] = src[index[i]] for i < n.
 hipError_t launch_gather(const uint32_t* src, const uint32_t* index, uint32_t n, uint32_t* dst,

@@ -105,7 +105,8 @@ struct Segment {
 
 struct Class {
   int S = 0;
-  uint32_t W = 0;  // 32-bit words per candidate stored in the pool
+  uint32_t W = 0;      // 32-bit words per candidate stored in the pool
+  uint32_t chunk = 0;  // candidates per side per iteration
   uint32_t cap = 0;
   std::unique_ptr<Columns> cols;
   uint32_t* d_pool = nullptr;
@@ -156,9 +157,14 @@ int make_class(pgx_server* s, int S, Class** out) {
   auto c = std::make_unique<Class>();
   c->S = S;
   c->W = (uint32_t)((S <= 31 ? 29 : 28) * S + 31) / 32;
+  // PRPs thin out as 1/bits (about 35/bits of sieved candidates are prime),
+  // so the chunk grows with the class width to decide most sides in one
+  // iteration: cfg.chunk at 512 bits, rounded up to a multiple of 16.
+  const uint32_t bits = (uint32_t)pgx::kernel_capacity_bits(S);
+  c->chunk = std::max<uint32_t>(16, (s->cfg.chunk * bits / 512 + 15) / 16 * 16);
   c->cap = s->cfg.pool_columns;
   c->cols = std::make_unique<Columns>(c->cap);
-  c->max_items = s->cfg.max_requests * 2u * s->cfg.chunk;
+  c->max_items = s->cfg.max_requests * 2u * c->chunk;
   c->cap_stage = std::max(s->cfg.stage_columns, s->cfg.max_side * 2u);
   hipError_t e = hipMalloc((void**)&c->d_pool, (size_t)c->W * c->cap * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&c->d_verdict, (size_t)c->cap * 4);
@@ -217,7 +223,7 @@ int launch_iteration(pgx_server* s, Class& c, bool copies_pending) {
     Request& r = *rp;
     for (int side = 0; side < 2; ++side) {
       if (r.done[side]) continue;
-      const uint32_t k = std::min(s->cfg.chunk, r.n_side[side] - r.cursor[side]);
+      const uint32_t k = std::min(c.chunk, r.n_side[side] - r.cursor[side]);
       const uint32_t base = r.col0 + (side ? r.n_side[0] : 0) + r.cursor[side];
       for (uint32_t j = 0; j < k; ++j) c.h_index[items + j] = base + j;
       c.segs.push_back(Segment{&r, side, items, k});

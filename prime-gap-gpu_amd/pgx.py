@@ -81,7 +81,21 @@ EXPORTED = (
     "pgx_serve_stats",
     "pgx_serve_destroy",
     "pgx_serve_last_error",
+    "pgx_submit_mk",
+    "pgx_build_candidates",
+    "pgx_search_create",
+    "pgx_search_destroy",
+    "pgx_search_sieve",
+    "pgx_search_test",
+    "pgx_search_step",
+    "pgx_search_tests",
+    "pgx_search_found",
+    "pgx_search_bitmap",
+    "pgx_search_last_error",
 )
+
+SEARCH_AS_WRITTEN = 1
+SEARCH_ACTIVE, SEARCH_COMPOSITES, SEARCH_FOUND = 0, 1, 2
 
 
 class Bucket(ctypes.Structure):
@@ -115,6 +129,12 @@ class ServeStats(ctypes.Structure):
     _fields_ = [("bytes_in_use", ctypes.c_uint64), ("tests", ctypes.c_uint64),
                 ("active", ctypes.c_uint32), ("waiting", ctypes.c_uint32),
                 ("in_flight", ctypes.c_uint32), ("finished_unread", ctypes.c_uint32)]
+
+
+class SearchConfig(ctypes.Structure):
+    _fields_ = [("P", ctypes.c_uint32), ("d", ctypes.c_uint32), ("m_start", ctypes.c_uint64),
+                ("m_inc", ctypes.c_uint32), ("max_prime", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32)]
 
 
 class Tuning(ctypes.Structure):
@@ -178,6 +198,23 @@ def _load() -> ctypes.CDLL:
                                    ctypes.POINTER(ctypes.c_uint32)]
     lib.pgx_serve_stats.argtypes = [vp, ctypes.POINTER(ServeStats)]
     lib.pgx_serve_destroy.argtypes = [vp]
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    lib.pgx_search_create.argtypes = [ctypes.POINTER(vp), vp, u32p, ctypes.c_uint32,
+                                      ctypes.POINTER(SearchConfig)]
+    lib.pgx_search_destroy.argtypes = [vp]
+    lib.pgx_search_sieve.argtypes = [vp, ctypes.c_uint32, u32p]
+    lib.pgx_search_test.argtypes = [vp, u32p]
+    lib.pgx_search_step.argtypes = [vp, ctypes.c_uint32, u32p, u32p]
+    lib.pgx_search_tests.argtypes = [vp, u32p, ctypes.c_size_t, szp]
+    lib.pgx_search_found.argtypes = [vp, u32p, u32p, ctypes.c_size_t, szp]
+    lib.pgx_search_bitmap.argtypes = [vp, ctypes.c_uint32, u32p, ctypes.c_size_t, szp]
+    lib.pgx_search_last_error.argtypes = [vp]
+    lib.pgx_search_last_error.restype = ctypes.c_char_p
+    lib.pgx_submit_mk.argtypes = [vp, u32p, ctypes.c_uint32, u64p, ctypes.c_size_t,
+                                  ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32]
+    lib.pgx_build_candidates.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32,
+                                         ctypes.c_int32, ctypes.c_uint32, vp, ctypes.c_uint32,
+                                         ctypes.c_uint32, vp, vp]
     lib.pgx_serve_last_error.argtypes = [vp]
     lib.pgx_serve_last_error.restype = ctypes.c_char_p
     return lib
@@ -226,6 +263,17 @@ def from_limbs(arr: np.ndarray) -> list[int]:
 
 
 # ---- candidate builder (fill_batch / init_K roles) -----------------------
+
+def primorial_over(P: int, d: int) -> int:
+    """K = P# / d (gap_common.cpp:60-65)."""
+    K = 1
+    for q in range(2, P + 1):
+        if all(q % r for r in range(2, int(q ** 0.5) + 1)):
+            K *= q
+    if K % d:
+        raise ValueError(f"{d} does not divide {P}#")
+    return K // d
+
 
 def k_bits(P: int, d: int) -> int:
     return int(lib.pgx_k_bits(P, d))
@@ -351,6 +399,35 @@ class Runner:
         self._pending = None
         return (v, r) if residues else v
 
+    def submit_mk(self, K: int, m: np.ndarray, x: int, sign: int = 1,
+                  mode: int = MODE_REFERENCE, residues: bool = False):
+        """Build N_i = m[i]*K + sign*x on the device (fill_batch,
+        gpu_testing.cu:171-172) and launch the test; ``wait`` returns as for
+        ``submit``."""
+        kw = max(1, (int(K).bit_length() + 31) // 32)
+        karr = to_limbs([int(K)], kw)[0]
+        marr = np.ascontiguousarray(m, dtype=np.uint64)
+        flags = mode | (WANT_RESIDUE if residues else 0)
+        self._check(lib.pgx_submit_mk(self._h, _u32(karr), kw, _u64(marr), len(marr), int(x),
+                                      int(sign), flags))
+        self._pending = (len(marr), residues, marr)
+
+    def run_mk(self, K: int, m: np.ndarray, x: int, sign: int = 1, mode: int = MODE_REFERENCE,
+               residues: bool = False):
+        self.submit_mk(K, m, x, sign, mode, residues)
+        return self.wait()
+
+    def build_candidates(self, K_ptr: int, k_words: int, m_ptr: int, x_ptr: int, x0: int,
+                         sign: int, n: int, out_ptr: int, ld: int, out_words: int,
+                         status_ptr: int, stream: int = 0) -> None:
+        """Device buffers only: column i of out = m[i]*K + sign*(x[i] or x0)."""
+        self._check(lib.pgx_build_candidates(self._h, ctypes.c_void_p(K_ptr), k_words,
+                                             ctypes.c_void_p(m_ptr),
+                                             ctypes.c_void_p(x_ptr or None), x0, sign, n,
+                                             ctypes.c_void_p(out_ptr), ld, out_words,
+                                             ctypes.c_void_p(status_ptr),
+                                             ctypes.c_void_p(stream or None)))
+
     def run_test(self, active: int, tests, results) -> None:
         """Reference semantics of run_test (miller_rabin.h:364-408): tests is the
         full batch of ints; the first min(active, len(tests)) are tested and
@@ -400,6 +477,86 @@ class Runner:
                                         max_bits, flags, ctypes.c_void_p(verdict_ptr),
                                         ctypes.c_void_p(residue_ptr or None), res_ld,
                                         ctypes.c_void_p(stream or None)))
+
+
+class Search:
+    """pgx_search_t: the m-interval search (sieve at X, test, bookkeeping) on
+    the device, one X at a time like the reference's sieve and GPU threads
+    (gap_search_gpu.cpp:462-804, gpu_testing.cu:104-354)."""
+
+    def __init__(self, runner: Runner, K: int, P: int, d: int, m_start: int, m_inc: int,
+                 max_prime: int, as_written: bool = False):
+        self.runner = runner
+        kw = max(1, (int(K).bit_length() + 31) // 32)
+        karr = to_limbs([int(K)], kw)[0]
+        cfg = SearchConfig(P, d, m_start, m_inc, max_prime, SEARCH_AS_WRITTEN if as_written else 0)
+        h = ctypes.c_void_p()
+        rc = lib.pgx_search_create(ctypes.byref(h), runner._h, _u32(karr), kw, ctypes.byref(cfg))
+        if rc != PGX_OK:
+            raise PgxError(rc, lib.pgx_last_error(runner._h).decode())
+        self._h = h
+        self.m_inc = m_inc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.pgx_search_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != PGX_OK:
+            raise PgxError(rc, lib.pgx_search_last_error(self._h).decode())
+
+    def sieve(self, X: int) -> int:
+        n = ctypes.c_uint32(0)
+        self._check(lib.pgx_search_sieve(self._h, X, ctypes.byref(n)))
+        return n.value
+
+    def test(self) -> int:
+        n = ctypes.c_uint32(0)
+        self._check(lib.pgx_search_test(self._h, ctypes.byref(n)))
+        return n.value
+
+    def step(self, X: int) -> tuple[int, int]:
+        nt, nf = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        self._check(lib.pgx_search_step(self._h, X, ctypes.byref(nt), ctypes.byref(nf)))
+        return nt.value, nf.value
+
+    def tests(self) -> np.ndarray:
+        n = ctypes.c_size_t(0)
+        self._check(lib.pgx_search_tests(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(1, n.value), dtype=np.uint32)
+        self._check(lib.pgx_search_tests(self._h, _u32(out), len(out), ctypes.byref(n)))
+        return out[:n.value]
+
+    def found(self) -> tuple[np.ndarray, np.ndarray]:
+        n = ctypes.c_size_t(0)
+        self._check(lib.pgx_search_found(self._h, None, None, 0, ctypes.byref(n)))
+        mi = np.zeros(max(1, n.value), dtype=np.uint32)
+        x = np.zeros(max(1, n.value), dtype=np.uint32)
+        self._check(lib.pgx_search_found(self._h, _u32(mi), _u32(x), len(mi), ctypes.byref(n)))
+        return mi[:n.value], x[:n.value]
+
+    def bitmap(self, which: int) -> np.ndarray:
+        n = ctypes.c_size_t(0)
+        self._check(lib.pgx_search_bitmap(self._h, which, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=np.uint32)
+        self._check(lib.pgx_search_bitmap(self._h, which, _u32(out), len(out), ctypes.byref(n)))
+        return out
+
+    def active(self) -> np.ndarray:
+        """Active m_i, ascending (expanded from the bitmap)."""
+        words = self.bitmap(SEARCH_ACTIVE)
+        bits = np.unpackbits(words.view(np.uint8), bitorder="little")[: self.m_inc // 2]
+        return (2 * np.nonzero(bits)[0] + 1).astype(np.uint32)
 
 
 class Plan:

@@ -75,8 +75,10 @@ def test_alone_and_mixed_equal_oracle(requests, expected):
         assert alone[i][:2] == want, (i, alone[i], want)
         assert mixed[1000 + i][:2] == want, (i, mixed[1000 + i], want)
         # early exit: a side stops within one chunk after its first PRP
+        # (chunk 16 at 512 bits, scaled up with the width class)
         n, p = len(requests[i][2]), len(requests[i][3])
-        bound = sum(min(s, ((f // 16) + 1) * 16) if f >= 0 else s
+        ch = 48 if requests[i][0] == 907 else 32
+        bound = sum(min(s, ((f // ch) + 1) * ch) if f >= 0 else s
                     for f, s in zip(want, (n, p)))
         assert alone[i][2] <= bound
 
