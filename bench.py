@@ -83,6 +83,16 @@ def time_ms(stream, fn, reps: int) -> float:
 
 MIXED = [(211, 16), (337, 16), (503, 24), (701, 32)]  # (P, buffer words), d = 210
 
+# HBM traffic of the headline kernel, from two rocprofv3 --pmc passes of this
+# script with --copy-probe (profiles/k_prp_plain_s17_p337_d2310_2e20.md):
+# KiB per dispatch of FETCH_SIZE and WRITE_SIZE, each scaled by known bytes /
+# counter reading of the 256 MiB device copy in the same run (copy read
+# 262144 KiB, FETCH_SIZE read 131087; copy wrote 262144 KiB, WRITE_SIZE 262144).
+TRAFFIC_KIB = {
+    "k_prp_plain<17,true>": {"fetch": 2195.90, "fetch_scale": 262144 / 131087.3,
+                             "write": 4352.0, "write_scale": 1.0},
+}
+
 
 def bench_mixed(stream, peak: float, reps: int) -> dict:
     """2^21 candidates, P in {211, 337, 503, 701}, d=210: one hipGraph launch
@@ -143,9 +153,9 @@ def bench_mixed(stream, peak: float, reps: int) -> dict:
 def bench_large(stream, peak: float, reps: int) -> dict:
     """P=907, d=30030 (~1260-bit N in 64-word buffers), 2^18 candidates:
     base-2 Fermat (persistent, autotuned), on-device compaction of the PRPs,
-    strong tests to bases 3 and 5 on the survivors only.  Pipelined as four
-    sub-batches: Fermat + compaction on one stream, base 3 and base 5 each on
-    their own stream, so sub-batch k's Miller-Rabin overlaps k+1's Fermat."""
+    then strong tests to bases 3 and 5 on the survivors only, the two bases
+    concurrently on two side streams (the survivors are ~3% of the batch, so
+    each base alone fills only a few percent of the SIMDs)."""
     import torch
 
     import oracle
@@ -163,31 +173,23 @@ def bench_large(stream, peak: float, reps: int) -> dict:
     t, rate = fermat.autotune(a.bucket(), pgx.MODE_FERMAT)
     log(f"large: tuned {t.as_dict()} {rate / 1e6:.3f} M/s")
     s3, s5 = torch.cuda.Stream(), torch.cuda.Stream()
-    SUB = 4
-    step = n // SUB
-
-    def sl(ptr, k):  # column offset k * step of a limb-interleaved buffer
-        return ptr + 4 * k * step
 
     def fermat_only():
-        fermat.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True,
+        fermat.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True, tuning=t,
                                stream=stream.cuda_stream)
 
     def pipeline():
-        for k in range(SUB):
-            fb = pgx.Bucket(sl(a.cand.data_ptr(), k), a.ld, a.limbs, step, a.max_bits,
-                            sl(a.verdict.data_ptr(), k), sl(a.res.data_ptr(), k), a.ld)
-            fermat.test_persistent(fb, pgx.MODE_FERMAT, residues=True, stream=stream.cuda_stream)
-            fermat.compact(fb.d_verdict, step, pgx.FLAG_FERMAT, sl(index.data_ptr(), k),
-                           count.data_ptr() + 4 * k, stream.cuda_stream)
-            ev = torch.cuda.Event()
-            ev.record(stream)
-            for run, st, vv in ((r3, s3, v3), (r5, s5, v5)):
-                st.wait_event(ev)
-                mb = pgx.Bucket(fb.d_cand, a.ld, a.limbs, step, a.max_bits,
-                                sl(vv.data_ptr(), k), 0, a.ld)
-                run.test_indexed(mb, sl(index.data_ptr(), k), count.data_ptr() + 4 * k,
-                                 pgx.MODE_STRONG, stream=st.cuda_stream)
+        fermat_only()
+        fermat.compact(a.verdict.data_ptr(), n, pgx.FLAG_FERMAT, index.data_ptr(),
+                       count.data_ptr(), stream.cuda_stream)
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        for run, st, vv in ((r3, s3, v3), (r5, s5, v5)):
+            st.wait_event(ev)
+            mb = pgx.Bucket(a.cand.data_ptr(), a.ld, a.limbs, n, a.max_bits, vv.data_ptr(), 0,
+                            a.ld)
+            run.test_indexed(mb, index.data_ptr(), count.data_ptr(), pgx.MODE_STRONG,
+                             stream=st.cuda_stream)
         for st in (s3, s5):  # join the side streams back for the timing events
             ev = torch.cuda.Event()
             ev.record(st)
@@ -198,7 +200,7 @@ def bench_large(stream, peak: float, reps: int) -> dict:
     v3.fill_(-1)
     v5.fill_(-1)
     ms_all = time_ms(stream, pipeline, reps)
-    k = int(count[:SUB].sum().item())
+    k = int(count[0].item())
     v = a.verdicts()
     f3 = v3.cpu().numpy().view(np.uint32)
     f5 = v5.cpu().numpy().view(np.uint32)
@@ -216,7 +218,7 @@ def bench_large(stream, peak: float, reps: int) -> dict:
     return {
         "workload": "P=907 d=30030, 2^18 sieved candidates (~1260 bits) in 64-word buffers: "
                     "Fermat base 2 -> device compaction -> strong bases 3 and 5 on survivors, "
-                    "4 sub-batches, Miller-Rabin on two side streams",
+                    "the two bases concurrently on two streams",
         "candidates": n, "bits": bits, "kernel_limbs": kernel_limbs(a.max_bits),
         "tuned": t.as_dict(),
         "fermat_ms": ms_fermat, "fermat_tests_per_s": n / (ms_fermat * 1e-3),
@@ -224,6 +226,59 @@ def bench_large(stream, peak: float, reps: int) -> dict:
         "survivors": k, "prp_all_bases": int(len(prp)),
         "fermat_valu_frac": work_per_test(bits) * n / (ms_fermat * 1e-3) / peak,
         "parity_sample_ok": ok,
+    }
+
+
+def bench_search(n_x: int = 8) -> dict:
+    """The m-interval search on the device: P=337, d=2310, m in [1e9, 1e9+2e7),
+    sieve of every X by the primes below 1e8, tests list, base-2 reference
+    PRP tests, found bookkeeping, for the first n_x X of the reference's
+    coprime X list.  CPU baseline leg: the oracle's restatement of the
+    reference's sieve (one thread) at the first X."""
+    import oracle
+    import pgx
+    P, d, m_start, m_inc, mp = 337, 2310, 10**9, 20_000_000, 100_000_000
+    K = pgx.primorial_over(P, d)
+    kw = pgx.to_limbs([K], (K.bit_length() + 31) // 32)[0]
+    xs = [int(x) for x in oracle.ms_coprime_x(P, d)[:n_x]]
+    run = pgx.Runner(1 << 20, limbs=16)
+    t0 = time.perf_counter()
+    s = pgx.Search(run, K, P, d, m_start, m_inc, mp)
+    t_setup = time.perf_counter() - t0
+    n_active = len(s.active())
+    # parity of the sieve at the first X (the oracle's C sieve, one thread)
+    comp, cpu_sieve_s = oracle.ms_sieve(kw, P, d, m_start, m_inc, mp, xs[0], timed=True)
+    n0 = s.sieve(xs[0])
+    ok = bool(np.array_equal(s.bitmap(pgx.SEARCH_COMPOSITES), comp))
+    s.test()
+    s.close()
+    s = pgx.Search(run, K, P, d, m_start, m_inc, mp)
+    sieve_s, test_s, tests, found = [], [], 0, 0
+    for X in xs:
+        t0 = time.perf_counter()
+        nt = s.sieve(X)
+        t1 = time.perf_counter()
+        nf = s.test()
+        t2 = time.perf_counter()
+        sieve_s.append(t1 - t0)
+        test_s.append(t2 - t1)
+        tests += nt
+        found += nf
+    s.close()
+    run.close()
+    wall = sum(sieve_s) + sum(test_s)
+    return {
+        "workload": f"m-interval search: P={P} d={d} m in [1e9, 1e9+2e7) ({n_active} active m), "
+                    f"sieve primes < 1e8, first {n_x} coprime X, base-2 PRP tests",
+        "setup_s": t_setup, "active_m": n_active, "tests": tests, "found": found,
+        "tests_first_x": n0,
+        "sieve_ms_per_x": 1e3 * float(np.mean(sieve_s)),
+        "test_ms_per_x": 1e3 * float(np.mean(test_s)),
+        "tests_per_s_incl_sieve": tests / wall,
+        "m_x_per_s": n_active * len(xs) / wall,
+        "cpu_sieve_ms_one_thread": 1e3 * cpu_sieve_s,  # per-X part, setup excluded
+        "sieve_speedup_vs_cpu_thread": cpu_sieve_s / float(np.mean(sieve_s)),
+        "parity_composites_first_x": ok,
     }
 
 
@@ -414,6 +469,12 @@ def main() -> int:
     # measured v_mad_u64_u32 issue rate of this device
     macs_launch = work_per_test(L) * BATCH
     achieved = macs_launch / (avg_launch_ms * 1e-3)
+    kname = f"k_prp_plain<{kernel_limbs(max_bits)},true>"
+    tk = TRAFFIC_KIB.get(kname)
+    traffic = (1024 * (tk["fetch"] * tk["fetch_scale"] + tk["write"] * tk["write_scale"])
+               if tk else None)
+    # algorithmic bytes per launch: candidates in, verdict words and residues out
+    bytes_launch = BATCH * (4 * LIMBS + 4 + 4 * LIMBS)
     if args.copy_probe:  # 256 MiB read + 256 MiB written by one copy kernel
         src = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
         dst = torch.empty_like(src)
@@ -429,6 +490,9 @@ def main() -> int:
     if not args.no_extras:
         extra["mixed_graph"] = bench_mixed(stream, peak, max(2, args.steps // 4))
         extra["large_fermat_mr"] = bench_large(stream, peak, max(2, args.steps // 4))
+        extra["search_interval"] = bench_search()
+        log(f"search: sieve {extra['search_interval']['sieve_ms_per_x']:.2f} ms/X, "
+            f"tests {extra['search_interval']['test_ms_per_x']:.2f} ms/X")
         srv = bench_serving()
         srv["plain_512_tests_per_s"] = tests_per_s
         extra["serving"] = srv
@@ -468,8 +532,10 @@ def main() -> int:
             "peak": peak / 1e12 if peak else None,
             "unit": "TMAC/s",
             "frac": achieved / peak if peak else None,
-            "traffic": None,
-            "kernel": f"k_prp_plain<{kernel_limbs(max_bits)},true>",
+            "traffic": traffic,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)",
+            "algorithmic_bytes_per_launch": bytes_launch,
+            "kernel": kname,
             "macs_per_launch": macs_launch,
             "avg_launch_ms": avg_launch_ms,
             "valu_rates_per_s": rate,

@@ -23,12 +23,13 @@
  * `as_written` selects the reference's arithmetic exactly as written for the
  * primes >= 100000: X * neg_inv_K is a 32-bit product (it wraps once it
  * reaches 2^32), and when X * max_prime >= m_start the start offset is not
- * halved before indexing the half-interval bitmap (:677-684), so those primes
+ * halved before indexing the half-interval bitmap (:671-689), so those primes
  * mark m_i = 2 t + 1 for t = f, f + p, ... where f is the full-interval
  * offset.  With as_written = 0 every prime marks exactly the m_i it divides.
  */
 #include <gmp.h>
 #include <stdint.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -113,26 +114,43 @@ int orc_ms_coprime_x(uint32_t P, uint32_t d, uint32_t* out, size_t cap, size_t* 
   return ORC_OK;
 }
 
-/* composites: m_inc/64 + 2 words (bit t <-> m_i = 2t + 1), cleared here. */
+static double seconds_now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* composites: m_inc/64 + 2 words (bit t <-> m_i = 2t + 1), cleared here.
+ * *mark_seconds (optional): time of the per-X work alone (clearing, start
+ * offsets, marking), i.e. without the prime list and the inverses of K, which
+ * the reference computes once per interval (gap_search_gpu.cpp:476-499). */
 int orc_ms_sieve(const uint32_t* K, uint32_t k_words, uint32_t P, uint32_t d, uint64_t m_start,
                  uint32_t m_inc, uint64_t max_prime, uint32_t X, int as_written,
-                 uint32_t* composites) {
+                 uint32_t* composites, double* mark_seconds) {
   if (!K || !k_words || (m_start & 1) || (m_inc & 1) || (d & 1) || (X & 1) || !composites ||
       max_prime > 2000000000ull)
     return ORC_E_ARG;
   const uint64_t half = m_inc / 2;
-  memset(composites, 0, (half / 32 + 2) * sizeof(uint32_t));
   mpz_t k;
   mpz_init(k);
   mpz_import(k, k_words, -1, sizeof(uint32_t), 0, 0, K);
   size_t np = 0;
   uint32_t* pr = odd_primes_below(max_prime, &np);
+  uint32_t* kmod = malloc((np ? np : 1) * sizeof(uint32_t));
+  uint32_t* nik = malloc((np ? np : 1) * sizeof(uint32_t));
+  for (size_t i = 0; i < np; ++i) {
+    const uint32_t p = pr[i];
+    kmod[i] = (uint32_t)mpz_fdiv_ui(k, p);
+    nik[i] = (p <= P) ? 0 : p - inverse_mod(kmod[i], p);
+  }
+  const double t_start = seconds_now();
+  memset(composites, 0, (half / 32 + 2) * sizeof(uint32_t));
   for (size_t i = 0; i < np; ++i) {
     const uint32_t p = pr[i];
     if (p <= P) {
       if (d % p) continue; /* divides K: X is chosen coprime to it */
       /* d wheel: the first odd m_i whose m K + X is divisible by p, then every p-th bit */
-      const uint64_t k_mod = mpz_fdiv_ui(k, p);
+      const uint64_t k_mod = kmod[i];
       const uint64_t shift = m_start % p;
       for (uint64_t mi = 1; mi < 2ull * p + 1; mi += 2)
         if (((shift + mi) * k_mod + X) % p == 0) {
@@ -141,8 +159,7 @@ int orc_ms_sieve(const uint32_t* K, uint32_t k_words, uint32_t P, uint32_t d, ui
         }
       continue;
     }
-    const uint32_t k_mod = (uint32_t)mpz_fdiv_ui(k, p);
-    const uint32_t neg_inv = p - inverse_mod(k_mod, p);
+    const uint32_t neg_inv = nik[i];
     /* f: the smallest odd offset with p | (m_start + f) K + X */
     uint64_t f, t0;
     if (p < SMALL_PRIME_LIMIT || !as_written) {
@@ -164,6 +181,9 @@ int orc_ms_sieve(const uint32_t* K, uint32_t k_words, uint32_t P, uint32_t d, ui
     }
     for (uint64_t t = t0; t < half; t += p) set_bit(composites, t);
   }
+  if (mark_seconds) *mark_seconds = seconds_now() - t_start;
+  free(kmod);
+  free(nik);
   free(pr);
   mpz_clear(k);
   return ORC_OK;

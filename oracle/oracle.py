@@ -58,7 +58,8 @@ def _load():
     lib.orc_ms_coprime_x.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_size_t, szp]
     lib.orc_ms_sieve.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                  ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
-                                 ctypes.c_uint32, ctypes.c_int, u32p]
+                                 ctypes.c_uint32, ctypes.c_int, u32p,
+                                 ctypes.POINTER(ctypes.c_double)]
     lib.orc_ms_step.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, u32p,
                                 u32p, szp, u32p, u32p, ctypes.c_uint32, u32p, szp, u32p, szp]
     return lib
@@ -168,15 +169,17 @@ def ms_coprime_x(P: int, d: int) -> np.ndarray:
 
 
 def ms_sieve(K_words: np.ndarray, P: int, d: int, m_start: int, m_inc: int, max_prime: int,
-             X: int, as_written: bool = False) -> np.ndarray:
-    """Composites bitmap (bit t <-> m_i = 2t + 1), m_inc/64 + 2 words."""
+             X: int, as_written: bool = False, timed: bool = False):
+    """Composites bitmap (bit t <-> m_i = 2t + 1), m_inc/64 + 2 words; with
+    timed, also the seconds of the per-X part (one thread)."""
     k = np.ascontiguousarray(K_words, dtype=np.uint32)
     out = np.zeros(bitmap_words(m_inc), dtype=np.uint32)
+    sec = ctypes.c_double(0)
     rc = lib.orc_ms_sieve(_u32(k), len(k), P, d, m_start, m_inc, max_prime, X, int(as_written),
-                          _u32(out))
+                          _u32(out), ctypes.byref(sec))
     if rc != 0:
         raise ValueError(f"orc_ms_sieve failed: {rc}")
-    return out
+    return (out, sec.value) if timed else out
 
 
 def ms_step(K_words, m_start: int, X: int, composites, active: np.ndarray, found: np.ndarray,

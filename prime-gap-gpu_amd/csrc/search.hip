@@ -30,7 +30,7 @@
 
 namespace {
 
-constexpr uint32_t kSmallPrimeLimit = 100000;  // gap_search_gpu.cpp:490
+constexpr uint32_t kSmallPrimeLimit = 100000;  // gap_search_gpu.cpp:495
 constexpr uint32_t kTileWords = 4096;
 constexpr uint32_t kTileBits = kTileWords * 32;
 constexpr uint32_t kTinyPrime = 64;

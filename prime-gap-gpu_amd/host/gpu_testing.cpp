@@ -8,14 +8,13 @@
 // What changes for MI355X:
 //  * GPU_BATCH_SIZE is 65536 (reference 8192, gpu_testing.cu:57-58): one lane
 //    owns one candidate, and 65536 lanes are one wavefront per SIMD on 256 CUs.
-//  * fill_batch writes N = (m_start + m_i) * K + x straight into the runner's
-//    limb staging (a 32x64-bit multiply-add over K's limbs), replacing
+//  * fill_batch only lists the multipliers m = m_start + m_i; the candidates
+//    N = m * K + x are built on the device (pgx_submit_mk), replacing
 //    mpz_mul_ui + mpz_add_ui per candidate (gpu_testing.cu:171-172) and the
-//    mpz_export pass in run_test (miller_rabin.h:373-378).  GPUBatch::z is not
-//    used.
-//  * the launch is asynchronous (pgx_submit); results are read by pgx_wait
-//    after the batch's bookkeeping, so host filling of the next batch overlaps
-//    the device.
+//    mpz_export pass in run_test (miller_rabin.h:373-378).  8 bytes per
+//    candidate cross PCIe instead of BITS/8.  GPUBatch::z is not used.
+//  * each of the GPU_BATCHES threads owns a runner (its own stream), so up to
+//    three batches run on the device at once, as the reference's threads do.
 //  * gpu_state_and_checks no longer asks for a recompile per BITS
 //    (gpu_testing.cu:369-385): the kernel limb count is chosen per batch from
 //    the candidates' bit length, up to PGX_MAX_BITS bits.
@@ -56,37 +55,12 @@ static const uint32_t kRounds = 1;  // base-2 strong test, as the reference's RO
 
 namespace {
 
-// K's limbs, little-endian, plus a spare limb for the product's carry-out.
+// K's limbs, little-endian.
 std::vector<uint32_t> limbs_of(const mpz_t K) {
-  std::vector<uint32_t> out((mpz_sizeinbase(K, 2) + 31) / 32 + 1, 0);
+  std::vector<uint32_t> out((mpz_sizeinbase(K, 2) + 31) / 32, 0);
   size_t words = 0;
   mpz_export(out.data(), &words, -1, sizeof(uint32_t), 0, 0, K);
   return out;
-}
-
-// dst (kLimbs words) = K * m + x; false if it does not fit.
-bool build_candidate(const std::vector<uint32_t>& K, uint64_t m, uint32_t x, uint32_t* dst) {
-  std::fill(dst, dst + kLimbs, 0u);
-  const uint32_t mh[2] = {(uint32_t)m, (uint32_t)(m >> 32)};
-  for (int h = 0; h < 2; ++h) {
-    if (!mh[h]) continue;
-    uint64_t carry = 0;
-    for (size_t j = 0; j < K.size(); ++j) {
-      const size_t k = j + h;
-      const uint64_t v = (uint64_t)K[j] * mh[h] + (k < kLimbs ? dst[k] : 0) + carry;
-      if (k < kLimbs) dst[k] = (uint32_t)v;
-      else if ((uint32_t)v) return false;
-      carry = v >> 32;
-    }
-    if (carry) return false;
-  }
-  uint64_t carry = x;
-  for (uint32_t k = 0; carry && k < kLimbs; ++k) {
-    const uint64_t v = (uint64_t)dst[k] + carry;
-    dst[k] = (uint32_t)v;
-    carry = v >> 32;
-  }
-  return carry == 0;
 }
 
 }  // namespace
@@ -113,10 +87,9 @@ static uint32_t process_finished_batch(TestData& test_data, GPUBatch& batch) {
   return found;
 }
 
-// Takes the next unknown m_i at test_data.testing_x into the batch and writes
-// their candidates into `staging` (test_data.lock must be held).
-static void fill_batch(TestData& test_data, GPUBatch& batch, const std::vector<uint32_t>& K,
-                       std::vector<uint32_t>& staging) {
+// Takes the next unknown m_i at test_data.testing_x into the batch and lists
+// their multipliers in `mult` (test_data.lock must be held).
+static void fill_batch(TestData& test_data, GPUBatch& batch, std::vector<uint64_t>& mult) {
   assert(batch.state == GPUBatch::EMPTY);
   assert(test_data.state == TestData::ACTIVE);
   std::fill(batch.active.begin(), batch.active.end(), 0);
@@ -127,8 +100,7 @@ static void fill_batch(TestData& test_data, GPUBatch& batch, const std::vector<u
     const uint32_t m_i = test_data.unknown_m_i[j];
     const uint32_t half = m_i >> 1;  // found_prime_m_i is a bitset over m_i / 2
     if (test_data.found_prime_m_i[half >> 5] & (1u << (half & 31))) continue;
-    if (!build_candidate(K, test_data.m_start + m_i, batch.x, &staging[n * kLimbs]))
-      throw std::runtime_error("m*K + x does not fit in GPU_BITS");
+    mult[n] = test_data.m_start + m_i;
     batch.active[n] = 1;
     batch.m_i[n] = m_i;
     n++;
@@ -143,7 +115,7 @@ void run_gpu_thread(int runner_num, int verbose, TestData& test_data, GPUBatch& 
   try {
     pthread_setname_np(pthread_self(), ("GPU(" + std::to_string(runner_num) + ")").c_str());
     const std::vector<uint32_t> K = limbs_of(K_in);
-    std::vector<uint32_t> staging(GPU_BATCH_SIZE * kLimbs);
+    std::vector<uint64_t> mult(GPU_BATCH_SIZE);
     std::vector<uint32_t> verdict(GPU_BATCH_SIZE);
     if (pgx_runner_create(&runner, 0, GPU_BATCH_SIZE, kLimbs, nullptr, kRounds) != PGX_OK)
       throw std::runtime_error(std::string("pgx_runner_create: ") + pgx_last_error(nullptr));
@@ -158,7 +130,7 @@ void run_gpu_thread(int runner_num, int verbose, TestData& test_data, GPUBatch& 
       test_data.lock();
       assert(!test_data.unknown_m_i.empty());
       batch.fill_start = high_resolution_clock::now();
-      fill_batch(test_data, batch, K, staging);
+      fill_batch(test_data, batch, mult);
       batch.fill_end = high_resolution_clock::now();
       if (batch.i == 0) {  // nothing left at this X: this batch retires
         assert(test_data.test_i == test_data.unknown_m_i.size());
@@ -177,7 +149,8 @@ void run_gpu_thread(int runner_num, int verbose, TestData& test_data, GPUBatch& 
       test_data.unlock();
 
       batch.gpu_start = high_resolution_clock::now();
-      if (pgx_submit(runner, staging.data(), batch.i, PGX_MODE_REFERENCE) != PGX_OK ||
+      if (pgx_submit_mk(runner, K.data(), (uint32_t)K.size(), mult.data(), batch.i, batch.x, 1,
+                        PGX_MODE_REFERENCE) != PGX_OK ||
           pgx_wait(runner, verdict.data(), nullptr) != PGX_OK)
         throw std::runtime_error(std::string("pgx: ") + pgx_last_error(runner));
       for (size_t i = 0; i < batch.i; i++) batch.result[i] = (verdict[i] == kRounds);
