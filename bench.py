@@ -24,6 +24,11 @@ import time
 
 import numpy as np
 
+# One hardware queue per stream: the serving engine runs each width class on
+# its own stream, and a class whose stream shares a hardware queue with a
+# wider class waits behind that class's kernels (set before HIP starts).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "prime-gap-gpu_amd"))
 
@@ -79,6 +84,21 @@ def time_ms(stream, fn, reps: int) -> float:
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def cpu_rate(cands: np.ndarray, cpu_seconds: float = 3.0) -> dict:
+    """GMP mpz_powm(2, N-1, N) tests/s on min(16, affinity) host threads over a
+    bounded sample of about cpu_seconds of CPU work (the oracle, timed as the
+    reported CPU baseline)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = min(16, len(os.sched_getaffinity(0)))
+    sec1, _ = oracle.time_powm2(cands[:32], threads=1, warmup=True)
+    per = sec1 / min(32, len(cands))
+    n = int(min(len(cands), max(256, cpu_seconds / per)))
+    sec, _ = oracle.time_powm2(cands[:n], threads=threads, warmup=False)
+    return {"tests_per_s": n / sec, "threads": threads, "sample": n,
+            "us_per_test_one_thread": per * 1e6}
 
 
 MIXED = [(211, 16), (337, 16), (503, 24), (701, 32)]  # (P, buffer words), d = 210
@@ -137,6 +157,8 @@ def bench_mixed(stream, peak: float, reps: int) -> dict:
     macs = sum(work_per_test(median_bits(L)) * len(L) for L in data)
     plan.close()
     runner.close()
+    cpu = [cpu_rate(L, 1.5) for L in data]
+    cpu_s = sum(len(L) / c["tests_per_s"] for L, c in zip(data, cpu))
     return {
         "workload": "P in {211,337,503,701}, d=210, 2^21 sieved candidates in 16/16/24/32-word "
                     "buffers, 4 buckets by kernel limb count, one hipGraph launch",
@@ -147,6 +169,7 @@ def bench_mixed(stream, peak: float, reps: int) -> dict:
         "valu_frac": macs / (ms_graph * 1e-3) / peak,
         "buckets": per_bucket,
         "parity_sample_ok": ok,
+        "cpu_tests_per_s": total / cpu_s, "cpu_threads": cpu[0]["threads"],
     }
 
 
@@ -195,8 +218,39 @@ def bench_large(stream, peak: float, reps: int) -> dict:
             ev.record(st)
             stream.wait_event(ev)
 
+    SUB = 8
+    step = n // SUB
+
+    def pipeline_sub():
+        """Sub-batches of plain Fermat launches: the MR tests of sub-batch k
+        run on the side streams while sub-batch k+1's Fermat runs."""
+        for j in range(SUB):
+            c0 = j * step
+            fermat.test_device(a.cand.data_ptr() + 4 * c0, a.ld, a.limbs, step, a.max_bits,
+                               pgx.MODE_FERMAT, a.verdict.data_ptr() + 4 * c0,
+                               a.res.data_ptr() + 4 * c0, a.ld, stream.cuda_stream)
+            fermat.compact(a.verdict.data_ptr() + 4 * c0, step, pgx.FLAG_FERMAT,
+                           index.data_ptr() + 4 * c0, count.data_ptr() + 4 * (1 + j),
+                           stream.cuda_stream)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            for run, st, vv in ((r3, s3, v3), (r5, s5, v5)):
+                st.wait_event(ev)
+                mb = pgx.Bucket(a.cand.data_ptr() + 4 * c0, a.ld, a.limbs, step, a.max_bits,
+                                vv.data_ptr() + 4 * c0, 0, a.ld)
+                run.test_indexed(mb, index.data_ptr() + 4 * c0, count.data_ptr() + 4 * (1 + j),
+                                 pgx.MODE_STRONG, stream=st.cuda_stream)
+        for st in (s3, s5):
+            ev = torch.cuda.Event()
+            ev.record(st)
+            stream.wait_event(ev)
+
+    assert n % SUB == 0
     pipeline()
     ms_fermat = time_ms(stream, fermat_only, reps)
+    pipeline_sub()
+    ms_sub = time_ms(stream, pipeline_sub, reps)
+    sub_ok = int(count[1:1 + SUB].sum().item())
     v3.fill_(-1)
     v5.fill_(-1)
     ms_all = time_ms(stream, pipeline, reps)
@@ -215,6 +269,9 @@ def bench_large(stream, peak: float, reps: int) -> dict:
     bits = median_bits(L)
     for r in (fermat, r3, r5):
         r.close()
+    cpu = cpu_rate(L, 3.0)
+    # CPU pipeline: Fermat on all, two more exponentiations per survivor
+    cpu_pipe = n / ((n + 2 * k) / cpu["tests_per_s"])
     return {
         "workload": "P=907 d=30030, 2^18 sieved candidates (~1260 bits) in 64-word buffers: "
                     "Fermat base 2 -> device compaction -> strong bases 3 and 5 on survivors, "
@@ -223,9 +280,12 @@ def bench_large(stream, peak: float, reps: int) -> dict:
         "tuned": t.as_dict(),
         "fermat_ms": ms_fermat, "fermat_tests_per_s": n / (ms_fermat * 1e-3),
         "pipeline_ms": ms_all, "pipeline_tests_per_s": n / (ms_all * 1e-3),
-        "survivors": k, "prp_all_bases": int(len(prp)),
+        "sub_batched_ms": ms_sub, "sub_batched_tests_per_s": n / (ms_sub * 1e-3),
+        "survivors": k, "survivors_sub_batched": sub_ok, "prp_all_bases": int(len(prp)),
         "fermat_valu_frac": work_per_test(bits) * n / (ms_fermat * 1e-3) / peak,
         "parity_sample_ok": ok,
+        "cpu_fermat_tests_per_s": cpu["tests_per_s"], "cpu_pipeline_tests_per_s": cpu_pipe,
+        "cpu_threads": cpu["threads"],
     }
 
 
@@ -250,6 +310,9 @@ def bench_search(n_x: int = 8) -> dict:
     comp, cpu_sieve_s = oracle.ms_sieve(kw, P, d, m_start, m_inc, mp, xs[0], timed=True)
     n0 = s.sieve(xs[0])
     ok = bool(np.array_equal(s.bitmap(pgx.SEARCH_COMPOSITES), comp))
+    t_list = s.tests()
+    sample = pgx.to_limbs([(m_start + int(mi)) * K + xs[0] for mi in t_list[:4096]], 16)
+    cpu = cpu_rate(sample, 2.0)
     s.test()
     s.close()
     s = pgx.Search(run, K, P, d, m_start, m_inc, mp)
@@ -278,6 +341,9 @@ def bench_search(n_x: int = 8) -> dict:
         "m_x_per_s": n_active * len(xs) / wall,
         "cpu_sieve_ms_one_thread": 1e3 * cpu_sieve_s,  # per-X part, setup excluded
         "sieve_speedup_vs_cpu_thread": cpu_sieve_s / float(np.mean(sieve_s)),
+        # CPU step at the first X: the sieve on one thread + the tests on the threads
+        "cpu_ms_first_x": 1e3 * (cpu_sieve_s + n0 / cpu["tests_per_s"]),
+        "gpu_ms_per_x": 1e3 * wall / len(xs), "cpu_threads": cpu["threads"],
         "parity_composites_first_x": ok,
     }
 
@@ -310,7 +376,7 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
     n_req = int(rate * duration_s)
     gaps = rng.exponential(1.0 / rate, size=n_req)
     pick = rng.integers(0, len(tmpl), size=n_req)
-    results, lat = {}, []
+    results, lat, lat_by_p = {}, [], {}
     with pgx.Server(hbm_budget=64 << 30, chunk=32) as srv:
         for k in range(len(sizes)):  # warm each width class (allocation, code load)
             P, bits, nxt, prv = tmpl[k * templates_per_p]
@@ -338,9 +404,22 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
                 if rid < n_req:
                     results[rid] = (nf, pf, tests)
                     lat.append(lns)
+                    lat_by_p.setdefault(tmpl[pick[rid]][0], []).append(lns)
         wall = (time.monotonic_ns() - start) * 1e-9
         tests = sum(r[2] for r in results.values())
     lat = np.array(lat) * 1e-6
+    # CPU leg: GMP tests of each side in ascending X until its first PRP, one
+    # thread, over a sample of request templates
+    t0 = time.perf_counter()
+    cpu_tests = 0
+    for k in sorted(check):
+        for side in (2, 3):
+            f = want[k][side - 2]
+            upto = len(tmpl[k][side]) if f < 0 else f + 1
+            oracle.verdicts(tmpl[k][side][:upto], oracle.MODE_REFERENCE, (2,))
+            cpu_tests += upto
+    cpu_req_s = len(check) / (time.perf_counter() - t0)
+    threads = min(16, len(os.sched_getaffinity(0)))
     consistent = all(results[r][:2] == want[pick[r]] for r in results if pick[r] in want)
     by_t = {}
     for r, v in results.items():
@@ -353,8 +432,14 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
         "requests": n_req, "wall_s": wall,
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "max_ms": float(lat.max()),
+        "p50_p99_ms_by_P": {str(P): [float(np.percentile(np.array(v) * 1e-6, 50)),
+                                     float(np.percentile(np.array(v) * 1e-6, 99))]
+                            for P, v in sorted(lat_by_p.items())},
         "tests": tests, "tests_per_s": tests / wall,
         "results_match_oracle_and_repeat": bool(consistent),
+        "cpu_requests_per_s_one_thread": cpu_req_s,
+        "cpu_requests_per_s": cpu_req_s * threads, "cpu_threads": threads,
+        "gpu_requests_per_s": n_req / wall,
     }
 
 
@@ -389,6 +474,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline plain-kernel workload")
+    ap.add_argument("--extras", default="mixed,large,search,serving",
+                    help="comma-separated extra configurations to run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -487,17 +574,21 @@ def main() -> int:
     log("VALU rates (lane-ops/s): " + ", ".join(f"{k}={v:.3e}" for k, v in rate.items() if v))
 
     extra = {}
-    if not args.no_extras:
+    todo = set() if args.no_extras else set(args.extras.split(","))
+    if "mixed" in todo:
         extra["mixed_graph"] = bench_mixed(stream, peak, max(2, args.steps // 4))
+    if "large" in todo:
         extra["large_fermat_mr"] = bench_large(stream, peak, max(2, args.steps // 4))
+    if "search" in todo:
         extra["search_interval"] = bench_search()
         log(f"search: sieve {extra['search_interval']['sieve_ms_per_x']:.2f} ms/X, "
             f"tests {extra['search_interval']['test_ms_per_x']:.2f} ms/X")
+    if "serving" in todo:
         srv = bench_serving()
         srv["plain_512_tests_per_s"] = tests_per_s
         extra["serving"] = srv
         log(f"serving: p50 {srv['p50_ms']:.2f} ms p99 {srv['p99_ms']:.2f} ms, "
-            f"{srv['tests_per_s'] / 1e6:.2f} M tests/s")
+            f"{srv['tests_per_s'] / 1e6:.2f} M tests/s, by P {srv['p50_p99_ms_by_P']}")
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -47,8 +47,8 @@ typedef struct pgx_runner pgx_runner_t;
  * 32-bit words each (the reference's BITS/32).  `bases` lists the Miller-Rabin
  * bases in test order (the reference passes generate_primes(rounds) = the
  * first `rounds` primes, miller_rabin.h:272-289); NULL with nbases = r means
- * the first r primes, as there.  Replaces test_runner_t's constructor,
- * miller_rabin.h:322-349. */
+ * the first r primes, as there.  Each base must be in [2, PGX_MAX_BASE].
+ * Replaces test_runner_t's constructor, miller_rabin.h:322-349. */
 int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t limbs,
                       const uint32_t* bases, uint32_t nbases);
 

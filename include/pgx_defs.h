@@ -28,6 +28,7 @@
 
 #define PGX_VERDICT_INVALID 0xFFFFFFFFu /* candidate rejected by the kernel */
 #define PGX_MAX_BASES 16
+#define PGX_MAX_BASE 255 /* largest Miller-Rabin base (the first 16 primes end at 53) */
 #define PGX_MAX_BITS 2068 /* widest candidate the kernels take (74 radix-2^28 limbs) */
 
 #endif /* PGX_DEFS_H */

@@ -11,6 +11,9 @@
 
 namespace {
 
+// mode bit selecting the folded squaring with shifted column sums
+constexpr uint32_t kColumnShift = 0x100u;
+
 template <int S>
 uint32_t one_candidate(const uint32_t* x, uint32_t limbs, uint32_t mode, const pgx::Bases& bases,
                        uint32_t* res_out) {
@@ -22,8 +25,12 @@ uint32_t one_candidate(const uint32_t* x, uint32_t limbs, uint32_t mode, const p
   uint32_t v = PGX_VERDICT_INVALID;
   if (spill == 0) {
     auto word = [&](int j) { return x[j]; };
-    v = (mode == PGX_MODE_FERMAT) ? pgx::prp_candidate<S, true>(w, word, mode, bases, rw)
-                                  : pgx::prp_candidate<S, false>(w, word, mode, bases, rw);
+    if (mode == (PGX_MODE_FERMAT | kColumnShift))  // the plain kernels' form at 12..24 limbs
+      v = pgx::prp_candidate<S, true, false>(w, word, PGX_MODE_FERMAT, bases, rw);
+    else if (mode == PGX_MODE_FERMAT)
+      v = pgx::prp_candidate<S, true>(w, word, mode, bases, rw);
+    else
+      v = pgx::prp_candidate<S, false>(w, word, mode, bases, rw);
   }
   if (res_out)
     for (uint32_t j = 0; j < limbs; ++j) res_out[j] = (j < (uint32_t)W) ? rw[j] : 0u;
@@ -50,9 +57,11 @@ const Table<PGX_EMU_S_VALUES> table;
 extern "C" {
 
 // Runs the kernel's per-candidate routine on the CPU with kernel limb count S
-// (radix limbs; 0 = the smallest built one that holds every candidate).  Candidate-major limbs in, verdict
-// words (and residues, candidate-major, if res != NULL) out.  Returns the S used,
-// or 0 if none is available.
+// (radix limbs; 0 = the smallest built one that holds every candidate).
+// Candidate-major limbs in, verdict words (and residues, candidate-major, if
+// res != NULL) out.  mode PGX_MODE_FERMAT | 0x100 runs the base-2 test with
+// the other form of the folded squaring.  Returns the S used, or 0 if none is
+// available.
 uint32_t pgx_lane_emu(const uint32_t* limbs_le, size_t n, uint32_t limbs, uint32_t S,
                       uint32_t mode, const uint32_t* bases, uint32_t nbases, uint32_t* verdict,
                       uint32_t* res) {

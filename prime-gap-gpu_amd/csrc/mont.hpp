@@ -104,40 +104,76 @@ PGX_FN void mont_sqr(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv) {
 // the square term by bit), so no separate doubling pass.  x < 2N in and out:
 // (2 * 4N^2 + R N) / R < 2N because R >= 8N.  Column sums stay below 2^64 for
 // RB = 28 (any S <= 80) and for RB = 29 up to S = 21 (1.5 S 2^59 < 2^64).
-template <int S>
+//
+// kPreShift: the shifts move onto the operands (2^bit x^2 = sum_{i<j}
+// (x_i << (1+bit)) x_j + sum_i (x_i << bit) x_i; limbs < 2^RB, so both fit 32
+// bits) and every column is a pure chain of multiply-adds, 2S shifts per
+// squaring instead of about three per column.  Otherwise the column sums are
+// shifted, which keeps more independent work at the start of a squaring.
+template <int S, bool kPreShift = true>
 PGX_FN void mont_sqr_dbl(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv, uint32_t bit) {
   constexpr int RB = Radix<S>::RB;
   constexpr uint32_t MASK = Radix<S>::MASK;
   static_assert(RB == 28 || S <= 21, "column bound of the folded doubling");
-  uint32_t m[S];
-  uint64_t carry = 0;
+  if constexpr (kPreShift) {
+    uint32_t xs[S], xd[S];
 #pragma unroll
-  for (int k = 0; k < 2 * S - 1; ++k) {
-    const int i0 = (k < S) ? 0 : k - S + 1;
-    uint64_t c0 = 0, c1 = 0;
+    for (int i = 0; i < S; ++i) {
+      xs[i] = x[i] << bit;
+      xd[i] = xs[i] << 1;
+    }
+    uint32_t m[S];
+    uint64_t carry = 0;
 #pragma unroll
-    for (int i = i0; i < k - i; ++i) {
-      if ((i - i0) & 1) c1 += mul64(x[i], x[k - i]);
-      else c0 += mul64(x[i], x[k - i]);
-    }
-    uint64_t t0 = carry, t1 = 0;
-    const int i1 = (k < S) ? k : S;
+    for (int k = 0; k < 2 * S - 1; ++k) {
+      const int i0 = (k < S) ? 0 : k - S + 1;
+      uint64_t t = carry;
 #pragma unroll
-    for (int i = i0; i < i1; ++i) {
-      if ((i - i0) & 1) t1 += mul64(m[i], n[k - i]);
-      else t0 += mul64(m[i], n[k - i]);
+      for (int i = i0; i < k - i; ++i) t += mul64(xd[i], x[k - i]);
+      if ((k & 1) == 0) t += mul64(xs[k >> 1], x[k >> 1]);
+      const int i1 = (k < S) ? k : S;
+#pragma unroll
+      for (int i = i0; i < i1; ++i) t += mul64(m[i], n[k - i]);
+      if (k < S) {
+        m[k] = ((uint32_t)t * ninv) & MASK;
+        t += mul64(m[k], n[0]);  // low RB bits become zero
+      } else {
+        x[k - S] = (uint32_t)t & MASK;
+      }
+      carry = t >> RB;
     }
-    uint64_t t = t0 + t1 + ((c0 + c1) << (1 + bit));
-    if ((k & 1) == 0) t += mul64(x[k >> 1], x[k >> 1]) << bit;
-    if (k < S) {
-      m[k] = ((uint32_t)t * ninv) & MASK;
-      t += mul64(m[k], n[0]);  // low RB bits become zero
-    } else {
-      x[k - S] = (uint32_t)t & MASK;
+    x[S - 1] = (uint32_t)carry;
+  } else {
+    uint32_t m[S];
+    uint64_t carry = 0;
+#pragma unroll
+    for (int k = 0; k < 2 * S - 1; ++k) {
+      const int i0 = (k < S) ? 0 : k - S + 1;
+      uint64_t c0 = 0, c1 = 0;
+#pragma unroll
+      for (int i = i0; i < k - i; ++i) {
+        if ((i - i0) & 1) c1 += mul64(x[i], x[k - i]);
+        else c0 += mul64(x[i], x[k - i]);
+      }
+      uint64_t t0 = carry, t1 = 0;
+      const int i1 = (k < S) ? k : S;
+#pragma unroll
+      for (int i = i0; i < i1; ++i) {
+        if ((i - i0) & 1) t1 += mul64(m[i], n[k - i]);
+        else t0 += mul64(m[i], n[k - i]);
+      }
+      uint64_t t = t0 + t1 + ((c0 + c1) << (1 + bit));
+      if ((k & 1) == 0) t += mul64(x[k >> 1], x[k >> 1]) << bit;
+      if (k < S) {
+        m[k] = ((uint32_t)t * ninv) & MASK;
+        t += mul64(m[k], n[0]);  // low RB bits become zero
+      } else {
+        x[k - S] = (uint32_t)t & MASK;
+      }
+      carry = t >> RB;
     }
-    carry = t >> RB;
+    x[S - 1] = (uint32_t)carry;
   }
-  x[S - 1] = (uint32_t)carry;
 }
 
 // x <- x * y / R mod N (x < 4N, y < 4N with x*y < 16N^2; result < 2N).
@@ -252,6 +288,39 @@ PGX_FN void add_full(uint32_t (&v)[S], const uint32_t (&w)[S], const uint32_t (&
     c = s >> RB;
   }
   sub_if_ge<S>(v, n);
+}
+
+// v scaled by 2^-L as a double: sum_j v_j 2^(RB j - L) (each term exact, S
+// rounded additions).  With L = bit length of N, N scales into [0.5, 1).
+template <int S>
+PGX_FN double scaled(const uint32_t (&v)[S], int L) {
+  constexpr int RB = Radix<S>::RB;
+  double d = 0.0;
+#pragma unroll
+  for (int j = 0; j < S; ++j) d += ldexp((double)v[j], RB * j - L);
+  return d;
+}
+
+// x <- a x mod N for an ordinary integer 2 <= a < 2^8, which keeps the
+// Montgomery form (the step of a set exponent bit for a small base): x < 2N in
+// (limbs < 2^RB), x < 2N out.  q = floor(a x / N) is estimated in double
+// precision (relative error below S 2^-52 <= 2^-45) and biased down by 2^-40,
+// so it is the true quotient or one less (a x / N < 2^9) and a x - q N lies in
+// [0, 2N).  One pass forms a x_j - q n_j with a signed carry.  inv_n =
+// 1 / scaled(n, L).
+template <int S>
+PGX_FN void mul_small(uint32_t (&x)[S], uint32_t a, const uint32_t (&n)[S], double inv_n, int L) {
+  constexpr int RB = Radix<S>::RB;
+  constexpr uint32_t MASK = Radix<S>::MASK;
+  const double qd = (double)a * scaled<S>(x, L) * inv_n * (1.0 - 0x1p-40);
+  const uint32_t q = (qd > 0.0) ? (uint32_t)qd : 0u;
+  int64_t c = 0;
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int64_t t = (int64_t)((uint64_t)x[j] * a) - (int64_t)((uint64_t)n[j] * q) + c;
+    x[j] = (uint32_t)t & MASK;
+    c = t >> RB;  // arithmetic: the floor of t / 2^RB
+  }
 }
 
 template <int S>

@@ -123,9 +123,9 @@ int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t lim
   h->bases.count = nbases;
   if (bases) {
     for (uint32_t k = 0; k < nbases; ++k) {
-      if (bases[k] < 2) {
+      if (bases[k] < 2 || bases[k] > PGX_MAX_BASE) {
         delete h;
-        return fail(nullptr, PGX_E_ARG, "base %u must be >= 2", bases[k]);
+        return fail(nullptr, PGX_E_ARG, "base %u must be in [2, %d]", bases[k], PGX_MAX_BASE);
       }
       h->bases.a[k] = bases[k];
     }

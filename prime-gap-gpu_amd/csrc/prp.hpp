@@ -38,23 +38,28 @@ struct Bases {
 };
 
 // Base 2: multiply by the base = doubling, folded into the squaring when the
-// radix leaves room for it (RB = 28, or RB = 29 with S <= 21).
-template <int S>
+// radix leaves room for it (RB = 28, or RB = 29 with S <= 21).  kPreShift
+// selects the folded squaring's form (mont_sqr_dbl).
+template <int S, bool kPreShift = true>
 struct MulBy2 {
   static constexpr bool kFolded = (Radix<S>::RB == 28) || (S <= 21);
+  static constexpr bool kShiftOperands = kPreShift;
   PGX_FN void operator()(uint32_t (&x)[S], const uint32_t (&)[S], uint32_t, uint32_t bit) const {
     dbl_lazy<S>(x, bit);
   }
 };
 
-// General base: Montgomery product by am = Mont(a) on set bits.
+// Small base a (2 <= a < 2^8): multiply by the ordinary integer a on set bits
+// (mul_small, about 11 S instructions instead of a Montgomery product).
 template <int S>
-struct MulByMont {
+struct MulBySmall {
   static constexpr bool kFolded = false;
-  const uint32_t (&am)[S];
-  PGX_FN void operator()(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv,
-                         uint32_t bit) const {
-    if (bit) mont_mul<S>(x, am, n, ninv);
+  static constexpr bool kShiftOperands = true;
+  uint32_t a;
+  double inv_n;
+  int L;
+  PGX_FN void operator()(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t, uint32_t bit) const {
+    if (bit) mul_small<S>(x, a, n, inv_n, L);
   }
 };
 
@@ -78,7 +83,7 @@ PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t 
     // bit p of N-1: bit p of N for p >= 1, zero for p == 0
     const uint32_t bit = (p > 0) ? ((w >> (p & 31)) & 1u) : 0u;
     if constexpr (Step::kFolded) {
-      mont_sqr_dbl<S>(x, n, ninv, bit);
+      mont_sqr_dbl<S, Step::kShiftOperands>(x, n, ninv, bit);
     } else {
       mont_sqr<S>(x, n, ninv);
       step(x, n, ninv, bit);
@@ -105,16 +110,16 @@ PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t 
   return f;
 }
 
-template <int S, class WordFn>
+template <int S, bool kPreShift, class WordFn>
 PGX_FN uint32_t test_base2(const uint32_t (&n)[S], uint32_t ninv, int L, int t, WordFn word,
                            uint32_t (&res)[S]) {
   uint32_t x[S];
   r_mod_n<S>(x, n, L);
   dbl_full<S>(x, n);  // Mont(2)
-  return powm_verdict<S>(x, n, ninv, L, t, word, MulBy2<S>{}, res);
+  return powm_verdict<S>(x, n, ninv, L, t, word, MulBy2<S, kPreShift>{}, res);
 }
 
-// General base a >= 2 (the extra Miller-Rabin bases).
+// Base 2 <= a < 2^8 (the extra Miller-Rabin bases).
 template <int S, class WordFn>
 PGX_FN uint32_t test_base(uint32_t a, const uint32_t (&n)[S], uint32_t ninv, int L, int t,
                           WordFn word, uint32_t (&res)[S]) {
@@ -133,13 +138,14 @@ PGX_FN uint32_t test_base(uint32_t a, const uint32_t (&n)[S], uint32_t ninv, int
   uint32_t x[S];
 #pragma unroll
   for (int j = 0; j < S; ++j) x[j] = am[j];
-  return powm_verdict<S>(x, n, ninv, L, t, word, MulByMont<S>{am}, res);
+  const double inv_n = 1.0 / scaled<S>(n, L);
+  return powm_verdict<S>(x, n, ninv, L, t, word, MulBySmall<S>{a, inv_n, L}, res);
 }
 
 // Whole test of one candidate given as W32 words (w).  Returns the verdict word
 // of `mode` (PGX_VERDICT_INVALID if N is even, <= 1 or wider than the kernel)
 // and writes the first base's a^(N-1) mod N as W32 words into res_w.
-template <int S, bool kBase2Only, class WordFn>
+template <int S, bool kBase2Only, bool kPreShift = true, class WordFn>
 PGX_FN uint32_t prp_candidate(const uint32_t (&w)[Radix<S>::W32], WordFn word, uint32_t mode,
                               const Bases& bases, uint32_t (&res_w)[Radix<S>::W32]) {
   constexpr int W = Radix<S>::W32;
@@ -154,14 +160,14 @@ PGX_FN uint32_t prp_candidate(const uint32_t (&w)[Radix<S>::W32], WordFn word, u
     from_words<S>(n, w);
     const uint32_t ninv = neg_inv32(w[0]);
     if (kBase2Only) {
-      v = test_base2<S>(n, ninv, L, t, word, r);
+      v = test_base2<S, kPreShift>(n, ninv, L, t, word, r);
     } else {
       const uint32_t need = (mode == PGX_MODE_STRONG) ? FLAG_SPRP : FLAG_QUIRK;
       uint32_t passed = 0;
       for (uint32_t b = 0; b < bases.count; ++b) {
         uint32_t f;
         if (b == 0) {
-          f = (bases.a[0] == 2u) ? test_base2<S>(n, ninv, L, t, word, r)
+          f = (bases.a[0] == 2u) ? test_base2<S, kPreShift>(n, ninv, L, t, word, r)
                                  : test_base<S>(bases.a[0], n, ninv, L, t, word, r);
         } else {
           uint32_t scratch[S];

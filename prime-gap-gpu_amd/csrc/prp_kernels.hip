@@ -18,8 +18,17 @@
 
 namespace pgx {
 
+// Form of the folded squaring per kernel (mont_sqr_dbl): operands shifted,
+// except in the one-wavefront-per-SIMD plain launches of 12..24 limbs, where
+// shifting the column sums measured faster (more independent work at the
+// start of each squaring when no other wavefront hides the latency).
+template <int S>
+constexpr bool plain_pre_shift() {
+  return !(S >= 12 && S <= 24);
+}
+
 // Tests the candidate in column p of a.cand and writes column p of the outputs.
-template <int S, bool kBase2Only>
+template <int S, bool kBase2Only, bool kPreShift>
 __device__ __forceinline__ void test_column(const PrpArgs& a, uint32_t p) {
   constexpr int W = Radix<S>::W32;
   const uint32_t* c = a.cand + p;
@@ -32,7 +41,8 @@ __device__ __forceinline__ void test_column(const PrpArgs& a, uint32_t p) {
   uint32_t rw[W];
   uint32_t v = PGX_VERDICT_INVALID;
   if (spill == 0) {
-    v = prp_candidate<S, kBase2Only>(w, [&](int j) { return c[j * ld]; }, a.mode, a.bases, rw);
+    v = prp_candidate<S, kBase2Only, kPreShift>(w, [&](int j) { return c[j * ld]; }, a.mode,
+                                                a.bases, rw);
   } else {
 #pragma unroll
     for (int j = 0; j < W; ++j) rw[j] = 0;
@@ -59,7 +69,7 @@ template <int S, bool kBase2Only>
 __global__ __launch_bounds__(kBlock) void k_prp_plain(PrpArgs a) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= work_items(a)) return;
-  test_column<S, kBase2Only>(a, a.index ? a.index[i] : i);
+  test_column<S, kBase2Only, plain_pre_shift<S>()>(a, a.index ? a.index[i] : i);
 }
 
 // Persistent: a fixed grid of resident wavefronts takes `chunk` work items at a
@@ -77,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_prp_persistent(PrpArgs a, uint32_t* 
     if (base >= n) break;
     for (uint32_t off = lane; off < chunk; off += 64) {
       const uint32_t i = base + off;
-      if (i < n) test_column<S, kBase2Only>(a, a.index ? a.index[i] : i);
+      if (i < n) test_column<S, kBase2Only, true>(a, a.index ? a.index[i] : i);
     }
   }
 }

@@ -13,10 +13,10 @@
 //   process_finished_batch        gpu_testing.cu:104-133  k_consume -> found bitmap F + list
 //   remove_vector                 :340-351             k_prune (A &= ~F)
 //
-// The tile kernel keeps a 16 KB slice of C in LDS: primes below 64 are laid
-// down as per-word patterns, primes below 16384 by the whole workgroup walking
-// their hits, the rest of the primes below 100000 by one lane each (LDS
-// atomics).  Primes from 100000 up hit a slice at most a few times, so each
+// The tile kernel keeps a 4 KB slice of C in LDS (enough workgroups to fill
+// the chip at 10^7 multipliers): primes below 64 are laid down as per-word
+// patterns, primes below 4096 by one wavefront each walking their hits, the
+// rest of the primes below 100000 by one lane each (LDS atomics).  Primes from 100000 up hit a slice at most a few times, so each
 // lane takes one prime and marks the whole interval with global atomics.
 #include <hip/hip_runtime.h>
 
@@ -31,12 +31,13 @@
 namespace {
 
 constexpr uint32_t kSmallPrimeLimit = 100000;  // gap_search_gpu.cpp:495
-constexpr uint32_t kTileWords = 4096;
+constexpr uint32_t kTileWords = 1024;
 constexpr uint32_t kTileBits = kTileWords * 32;
 constexpr uint32_t kTinyPrime = 64;
 constexpr uint32_t kCoopPrime = kTileBits / 8;
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kConsumeBlock = 1024;
 
 // Odd primes below `limit` by a segmented sieve of Eratosthenes (the
 // reference iterates primesieve, gap_search_gpu.cpp:481-499).
@@ -95,9 +96,21 @@ __global__ __launch_bounds__(kBlock) void k_prep(const uint32_t* __restrict__ K,
   ms_mod[i] = (uint32_t)(m_start % q);
 }
 
+// v mod q for v < 2^52: the quotient from a double-precision product (off
+// by at most one), then one correction; far cheaper than a 64-bit integer
+// division on the device.
+__device__ __forceinline__ uint32_t mod_u52(uint64_t v, uint32_t q) {
+  const uint64_t g = (uint64_t)((double)v * (1.0 / (double)q));
+  int64_t r = (int64_t)(v - g * q);
+  if (r < 0) r += q;
+  else if (r >= (int64_t)q) r -= q;
+  return (uint32_t)r;
+}
+
 // Smallest odd offset f with q | (m_start + f) K + X, halved: the first bit.
 __device__ __forceinline__ uint32_t first_bit(uint32_t q, uint32_t nik, uint32_t msm, uint32_t X) {
-  uint32_t f = (uint32_t)(((uint64_t)X * nik + (q - msm)) % q);
+  const uint64_t v = (uint64_t)X * nik + (q - msm);  // < 2^52 for X < 2^20
+  uint32_t f = (v >> 52) ? (uint32_t)(v % q) : mod_u52(v, q);
   if (!(f & 1u)) f += q;
   return f >> 1;
 }
@@ -142,9 +155,10 @@ __global__ __launch_bounds__(kBlock) void k_sieve_tiles(SmallPrimes sp, uint32_t
   }
   __syncthreads();
   if (lo < half) {
-    for (uint32_t j = sp.n_tiny; j < sp.n_coop; ++j) {
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t j = sp.n_tiny + wave; j < sp.n_coop; j += kBlock / 64) {
       const uint32_t q = sp.p[j];
-      for (uint32_t t = first_at_or_above(sp.start[j], q, lo) + tid * q; t < hi; t += kBlock * q)
+      for (uint32_t t = first_at_or_above(sp.start[j], q, lo) + lane * q; t < hi; t += 64 * q)
         atomicOr(&tile[(t - lo) >> 5], 1u << (t & 31u));
     }
     for (uint32_t j = sp.n_coop + tid; j < sp.n_small; j += kBlock) {
@@ -264,24 +278,35 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ A,
 }
 
 // Verdicts of one batch: m_i passing every base joins F and the found list.
-__global__ __launch_bounds__(kBlock) void k_consume(const uint32_t* __restrict__ verdict,
-                                                    const uint32_t* __restrict__ tests, uint32_t n,
-                                                    uint32_t rounds, uint32_t X,
-                                                    uint32_t* __restrict__ F,
-                                                    uint32_t* __restrict__ found_mi,
-                                                    uint32_t* __restrict__ found_x,
-                                                    uint32_t* __restrict__ found_count) {
+// One atomic per workgroup reserves the list slots (a single counter taking
+// one atomic per wavefront serialises at this rate of hits).
+__global__ __launch_bounds__(kConsumeBlock) void k_consume(
+    const uint32_t* __restrict__ verdict, const uint32_t* __restrict__ tests, uint32_t n,
+    uint32_t rounds, uint32_t X, uint32_t* __restrict__ F, uint32_t* __restrict__ found_mi,
+    uint32_t* __restrict__ found_x, uint32_t* __restrict__ found_count) {
+  __shared__ uint32_t wave_count[kConsumeBlock / 64];
+  __shared__ uint32_t block_base;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool hit = (i < n) && verdict[i] == rounds;
   const uint64_t ballot = __ballot(hit);
-  const uint32_t lane = threadIdx.x & 63u;
-  uint32_t base = 0;
-  if (lane == 0 && ballot) base = atomicAdd(found_count, (uint32_t)__popcll(ballot));
-  base = __builtin_amdgcn_readfirstlane(base);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (lane == 0) wave_count[wave] = (uint32_t)__popcll(ballot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (uint32_t w = 0; w < kConsumeBlock / 64; ++w) {
+      const uint32_t c = wave_count[w];
+      wave_count[w] = total;
+      total += c;
+    }
+    block_base = total ? atomicAdd(found_count, total) : 0u;
+  }
+  __syncthreads();
   if (hit) {
     const uint32_t mi = tests[i], t = mi >> 1;
     atomicOr(&F[t >> 5], 1u << (t & 31u));
-    const uint32_t pos = base + (uint32_t)__popcll(ballot & ((1ull << lane) - 1ull));
+    const uint32_t pos = block_base + wave_count[wave] +
+                         (uint32_t)__popcll(ballot & ((1ull << lane) - 1ull));
     found_mi[pos] = mi;
     found_x[pos] = X;
   }
@@ -588,7 +613,8 @@ int pgx_search_test(pgx_search_t* s, uint32_t* n_found) {
     int rc = pgx::launch_prp(h, h->d_cand, nb, h->limbs, nb, s->max_bits, PGX_MODE_REFERENCE,
                              h->d_verdict, nullptr, nb, st);
     if (rc != PGX_OK) return sfail(s, rc, h->err.c_str());
-    hipLaunchKernelGGL(k_consume, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL(k_consume, dim3((nb + kConsumeBlock - 1) / kConsumeBlock),
+                       dim3(kConsumeBlock), 0, st,
                        h->d_verdict, s->d_tests + off, nb, (uint32_t)h->bases.count, s->cur_x,
                        s->d_F, s->d_found_mi, s->d_found_x, s->d_words + 1);
     PGX_SHIP(s, hipGetLastError());

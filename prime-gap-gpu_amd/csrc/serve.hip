@@ -174,7 +174,14 @@ int make_class(pgx_server* s, int S, Class** out) {
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_out, (size_t)c->max_items * 4, 0);
   if (e == hipSuccess)
     e = hipHostMalloc((void**)&c->h_stage, (size_t)c->W * c->cap_stage * 4, 0);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    // A stream created with a CU mask gets a hardware queue of its own, so a
+    // narrow class never waits behind a wide class's kernels in a queue that
+    // the runtime shares between streams.  The mask holds every CU.
+    std::vector<uint32_t> mask((s->h->n_cu + 31) / 32, 0u);
+    for (int cu = 0; cu < s->h->n_cu; ++cu) mask[cu / 32] |= 1u << (cu % 32);
+    e = hipExtStreamCreateWithCUMask(&c->stream, (uint32_t)mask.size(), mask.data());
+  }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->done, hipEventDisableTiming);
   if (e != hipSuccess) {
     release_class(*c);

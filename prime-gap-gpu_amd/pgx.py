@@ -54,6 +54,7 @@ FLAG_SPRP = 2
 FLAG_QUIRK = 4
 VERDICT_INVALID = 0xFFFFFFFF
 MAX_BITS = 2068
+MAX_BASE = 255
 
 EXPORTED = (
     "pgx_runner_create",

@@ -157,3 +157,15 @@ def test_bad_tuning_and_bucket_are_rejected():
         with pytest.raises(pgx.PgxError) as e:
             run.test_persistent(wide, pgx.MODE_FERMAT)
         assert e.value.code == pgx.PGX_E_INPUT
+
+
+def test_base_range_is_checked():
+    with pytest.raises(pgx.PgxError) as e:
+        pgx.Runner(16, limbs=16, bases=[2, 256])
+    assert e.value.code == pgx.PGX_E_ARG
+    with pytest.raises(pgx.PgxError):
+        pgx.Runner(16, limbs=16, bases=[1])
+    with pgx.Runner(16, limbs=16, bases=[2, 255]) as r:  # the largest base works
+        L = pgx.to_limbs([1000003, 3215031751, 2047], 16)
+        v = r.run(L, pgx.MODE_STRONG)
+        assert np.array_equal(v, O.verdicts(L, O.MODE_STRONG, (2, 255)))

@@ -149,10 +149,11 @@ def test_lane_routine_fixture_records(emu, vectors):
 def test_lane_routine_sieved_vs_oracle(emu):
     m = pgx.gen_multipliers(0x5EED0001, 10**7, 10**7 + 10**4, 2310, 2)
     L, _, _ = pgx.gen_sieved(337, 2310, m, 200000, 200, 16)
+    ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
     for S in (17, 20):  # 462-bit candidates: 17 limbs is the narrowest that holds them
-        v, r = emu(L, S, pgx.MODE_FERMAT)
-        ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
-        assert np.array_equal(v, ov) and np.array_equal(r, orr)
+        for mode in (pgx.MODE_FERMAT, pgx.MODE_FERMAT | 0x100):  # both folded-squaring forms
+            v, r = emu(L, S, mode)
+            assert np.array_equal(v, ov) and np.array_equal(r, orr), (S, mode)
     v, _ = emu(L, 17, pgx.MODE_REFERENCE, (2, 3, 5))
     assert np.array_equal(v, O.verdicts(L, O.MODE_REFERENCE, (2, 3, 5)))
 
@@ -164,3 +165,22 @@ def test_lane_routine_rejects_what_does_not_fit(emu):
     v, _ = emu(L, 1, pgx.MODE_FERMAT)
     assert list(v[:3]) == [pgx.VERDICT_INVALID] * 3
     assert v[3] != pgx.VERDICT_INVALID
+
+
+def test_lane_routine_small_bases_tiny_n(emu):
+    """Miller-Rabin bases other than 2 (multiply-by-a steps) on every odd
+    N < 4096, in kernels much wider than N, against CPython's pow."""
+    Ns = list(range(3, 4096, 2))
+    L = pgx.to_limbs(Ns, 4)
+    bases = (3, 5, 251)
+    want = []
+    for N in Ns:
+        passed = 0
+        for a in bases:
+            if a % N == 0 or not O.py_ref_verdict(N, a):
+                break
+            passed += 1
+        want.append(passed)
+    for S in (1, 2, 3, 10):
+        v, _ = emu(L, S, pgx.MODE_REFERENCE, bases)
+        assert v.tolist() == want, S
