@@ -173,6 +173,32 @@ def bench_mixed(stream, peak: float, reps: int) -> dict:
     }
 
 
+def bench_persistent(cands: np.ndarray, stream, peak: float, bits: int, reps: int) -> dict:
+    """The headline's 2^20 candidates in one limb-interleaved buffer, tested by
+    one autotuned persistent launch per step instead of 16 plain launches."""
+    import pgx
+    a = pgx.DeviceBatch(cands)
+    run = pgx.Runner(1, limbs=LIMBS)
+    t, _ = run.autotune(a.bucket(), pgx.MODE_FERMAT)
+
+    def step():
+        run.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True, tuning=t,
+                            stream=stream.cuda_stream)
+    step()
+    ms = time_ms(stream, step, reps)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    ov = oracle.verdicts(cands[:1024], oracle.MODE_FERMAT)
+    ok = bool(np.array_equal(a.verdicts()[:1024], ov))
+    run.close()
+    n = len(cands)
+    return {
+        "workload": "the headline's 2^20 candidates, one persistent launch per step",
+        "tuned": t.as_dict(), "ms_per_step": ms, "tests_per_s": n / (ms * 1e-3),
+        "valu_frac": work_per_test(bits) * n / (ms * 1e-3) / peak, "parity_sample_ok": ok,
+    }
+
+
 def bench_large(stream, peak: float, reps: int) -> dict:
     """P=907, d=30030 (~1260-bit N in 64-word buffers), 2^18 candidates:
     base-2 Fermat (persistent, autotuned), on-device compaction of the PRPs,
@@ -218,39 +244,8 @@ def bench_large(stream, peak: float, reps: int) -> dict:
             ev.record(st)
             stream.wait_event(ev)
 
-    SUB = 8
-    step = n // SUB
-
-    def pipeline_sub():
-        """Sub-batches of plain Fermat launches: the MR tests of sub-batch k
-        run on the side streams while sub-batch k+1's Fermat runs."""
-        for j in range(SUB):
-            c0 = j * step
-            fermat.test_device(a.cand.data_ptr() + 4 * c0, a.ld, a.limbs, step, a.max_bits,
-                               pgx.MODE_FERMAT, a.verdict.data_ptr() + 4 * c0,
-                               a.res.data_ptr() + 4 * c0, a.ld, stream.cuda_stream)
-            fermat.compact(a.verdict.data_ptr() + 4 * c0, step, pgx.FLAG_FERMAT,
-                           index.data_ptr() + 4 * c0, count.data_ptr() + 4 * (1 + j),
-                           stream.cuda_stream)
-            ev = torch.cuda.Event()
-            ev.record(stream)
-            for run, st, vv in ((r3, s3, v3), (r5, s5, v5)):
-                st.wait_event(ev)
-                mb = pgx.Bucket(a.cand.data_ptr() + 4 * c0, a.ld, a.limbs, step, a.max_bits,
-                                vv.data_ptr() + 4 * c0, 0, a.ld)
-                run.test_indexed(mb, index.data_ptr() + 4 * c0, count.data_ptr() + 4 * (1 + j),
-                                 pgx.MODE_STRONG, stream=st.cuda_stream)
-        for st in (s3, s5):
-            ev = torch.cuda.Event()
-            ev.record(st)
-            stream.wait_event(ev)
-
-    assert n % SUB == 0
     pipeline()
     ms_fermat = time_ms(stream, fermat_only, reps)
-    pipeline_sub()
-    ms_sub = time_ms(stream, pipeline_sub, reps)
-    sub_ok = int(count[1:1 + SUB].sum().item())
     v3.fill_(-1)
     v5.fill_(-1)
     ms_all = time_ms(stream, pipeline, reps)
@@ -280,8 +275,7 @@ def bench_large(stream, peak: float, reps: int) -> dict:
         "tuned": t.as_dict(),
         "fermat_ms": ms_fermat, "fermat_tests_per_s": n / (ms_fermat * 1e-3),
         "pipeline_ms": ms_all, "pipeline_tests_per_s": n / (ms_all * 1e-3),
-        "sub_batched_ms": ms_sub, "sub_batched_tests_per_s": n / (ms_sub * 1e-3),
-        "survivors": k, "survivors_sub_batched": sub_ok, "prp_all_bases": int(len(prp)),
+        "survivors": k, "prp_all_bases": int(len(prp)),
         "fermat_valu_frac": work_per_test(bits) * n / (ms_fermat * 1e-3) / peak,
         "parity_sample_ok": ok,
         "cpu_fermat_tests_per_s": cpu["tests_per_s"], "cpu_pipeline_tests_per_s": cpu_pipe,
@@ -474,7 +468,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline plain-kernel workload")
-    ap.add_argument("--extras", default="mixed,large,search,serving",
+    ap.add_argument("--extras", default="persistent,mixed,large,search,serving",
                     help="comma-separated extra configurations to run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known "
@@ -575,6 +569,10 @@ def main() -> int:
 
     extra = {}
     todo = set() if args.no_extras else set(args.extras.split(","))
+    if "persistent" in todo:
+        extra["same_workload_persistent"] = bench_persistent(cands, stream, peak, L,
+                                                             max(2, args.steps // 4))
+        log(f"persistent: {extra['same_workload_persistent']['tests_per_s'] / 1e6:.2f} M tests/s")
     if "mixed" in todo:
         extra["mixed_graph"] = bench_mixed(stream, peak, max(2, args.steps // 4))
     if "large" in todo:
