@@ -227,6 +227,8 @@ def bench_large(stream, peak: float, reps: int) -> dict:
         fermat.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True, tuning=t,
                                stream=stream.cuda_stream)
 
+    mr_lanes = [1]
+
     def pipeline():
         fermat_only()
         fermat.compact(a.verdict.data_ptr(), n, pgx.FLAG_FERMAT, index.data_ptr(),
@@ -238,7 +240,7 @@ def bench_large(stream, peak: float, reps: int) -> dict:
             mb = pgx.Bucket(a.cand.data_ptr(), a.ld, a.limbs, n, a.max_bits, vv.data_ptr(), 0,
                             a.ld)
             run.test_indexed(mb, index.data_ptr(), count.data_ptr(), pgx.MODE_STRONG,
-                             stream=st.cuda_stream)
+                             tuning=pgx.Tuning(8, 4, 64, mr_lanes[0]), stream=st.cuda_stream)
         for st in (s3, s5):  # join the side streams back for the timing events
             ev = torch.cuda.Event()
             ev.record(st)
@@ -246,6 +248,12 @@ def bench_large(stream, peak: float, reps: int) -> dict:
 
     pipeline()
     ms_fermat = time_ms(stream, fermat_only, reps)
+    by_lanes = {}
+    for lanes in (1, 2, 4):  # lanes per candidate of the Miller-Rabin stage
+        mr_lanes[0] = lanes
+        pipeline()
+        by_lanes[lanes] = time_ms(stream, pipeline, reps)
+    mr_lanes[0] = min(by_lanes, key=by_lanes.get)
     v3.fill_(-1)
     v5.fill_(-1)
     ms_all = time_ms(stream, pipeline, reps)
@@ -275,6 +283,8 @@ def bench_large(stream, peak: float, reps: int) -> dict:
         "tuned": t.as_dict(),
         "fermat_ms": ms_fermat, "fermat_tests_per_s": n / (ms_fermat * 1e-3),
         "pipeline_ms": ms_all, "pipeline_tests_per_s": n / (ms_all * 1e-3),
+        "mr_lanes_per_candidate": mr_lanes[0],
+        "pipeline_ms_by_mr_lanes": {str(k): v for k, v in by_lanes.items()},
         "survivors": k, "prp_all_bases": int(len(prp)),
         "fermat_valu_frac": work_per_test(bits) * n / (ms_fermat * 1e-3) / peak,
         "parity_sample_ok": ok,
@@ -342,7 +352,9 @@ def bench_search(n_x: int = 8) -> dict:
     }
 
 
-def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: int = 100) -> dict:
+def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: int = 100,
+                  group_min_bits: int = 0, group_max_items: int = 0,
+                  group_lanes: int = 4) -> dict:
     """Poisson stream of per-m next/prev-prime requests (P in {337, 503, 907},
     d=2310, 20-500 sieved X per side), continuous batching under a 64 GB cap.
     Each request is submitted at its scheduled arrival time; latency runs from
@@ -371,7 +383,8 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
     gaps = rng.exponential(1.0 / rate, size=n_req)
     pick = rng.integers(0, len(tmpl), size=n_req)
     results, lat, lat_by_p = {}, [], {}
-    with pgx.Server(hbm_budget=64 << 30, chunk=32) as srv:
+    with pgx.Server(hbm_budget=64 << 30, chunk=32, group_min_bits=group_min_bits,
+                    group_max_items=group_max_items, group_lanes=group_lanes) as srv:
         for k in range(len(sizes)):  # warm each width class (allocation, code load)
             P, bits, nxt, prv = tmpl[k * templates_per_p]
             srv.submit(10**9 + k, nxt, prv, bits)
@@ -381,6 +394,9 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
             warm += len(srv.poll())
             if time.time() - t_warm > 60:
                 raise RuntimeError(f"serving warm-up did not finish: {srv.stats()}")
+        import gc
+        gc.collect()
+        gc.disable()  # a collector pause of this driver would count as engine latency
         start = time.monotonic_ns()
         arrivals = start + np.cumsum(gaps * 1e9).astype(np.int64)
         i = 0
@@ -400,6 +416,7 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
                     lat.append(lns)
                     lat_by_p.setdefault(tmpl[pick[rid]][0], []).append(lns)
         wall = (time.monotonic_ns() - start) * 1e-9
+        gc.enable()
         tests = sum(r[2] for r in results.values())
     lat = np.array(lat) * 1e-6
     # CPU leg: GMP tests of each side in ascending X until its first PRP, one
@@ -434,6 +451,8 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
         "cpu_requests_per_s_one_thread": cpu_req_s,
         "cpu_requests_per_s": cpu_req_s * threads, "cpu_threads": threads,
         "gpu_requests_per_s": n_req / wall,
+        "group_min_bits": group_min_bits, "group_max_items": group_max_items,
+        "group_lanes": group_lanes,
     }
 
 
@@ -470,6 +489,10 @@ def main() -> int:
                     help="only the headline plain-kernel workload")
     ap.add_argument("--extras", default="persistent,mixed,large,search,serving",
                     help="comma-separated extra configurations to run")
+    ap.add_argument("--serve-group", default="0,0,4",
+                    help="bits,items,lanes: serving iterations of classes at least `bits` wide "
+                         "with at most `items` candidates use `lanes` lanes per candidate "
+                         "(0,0,4 = never)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -582,7 +605,13 @@ def main() -> int:
         log(f"search: sieve {extra['search_interval']['sieve_ms_per_x']:.2f} ms/X, "
             f"tests {extra['search_interval']['test_ms_per_x']:.2f} ms/X")
     if "serving" in todo:
-        srv = bench_serving()
+        gb, gi, gl = (int(v) for v in args.serve_group.split(","))
+        srv = bench_serving(group_min_bits=gb, group_max_items=gi, group_lanes=gl)
+        # the same stream with 4 lanes per candidate for the ~1300-bit class
+        wide = bench_serving(group_min_bits=1000, group_max_items=32768, group_lanes=4)
+        srv["with_4_lane_groups_over_1000_bits"] = {
+            k: wide[k] for k in ("p50_ms", "p99_ms", "p50_p99_ms_by_P", "tests_per_s",
+                                 "results_match_oracle_and_repeat")}
         srv["plain_512_tests_per_s"] = tests_per_s
         extra["serving"] = srv
         log(f"serving: p50 {srv['p50_ms']:.2f} ms p99 {srv['p99_ms']:.2f} ms, "

@@ -280,6 +280,10 @@ typedef struct {
   uint32_t pool_columns; /* candidate columns per width class pool */
   uint32_t max_side;     /* most candidates one side may carry */
   uint32_t stage_columns;/* columns admitted per class per iteration (pinned staging) */
+  uint32_t group_min_bits;  /* classes at least this wide (kernel capacity, bits) test ... */
+  uint32_t group_max_items; /* ... iterations of at most this many candidates with ... */
+  uint32_t group_lanes;     /* ... 2 or 4 lanes per candidate (lower latency for the wide
+                               classes on a lightly loaded GPU); group_max_items 0 = never */
 } pgx_serve_config_t;
 
 typedef struct {

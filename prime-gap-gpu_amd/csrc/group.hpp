@@ -1,0 +1,345 @@
+// Several lanes per candidate: the PRP test of one number spread over a group
+// of G = 2 or 4 consecutive lanes (inside one quad), for launches with few
+// candidates, where one lane per candidate leaves most SIMDs idle and the
+// latency of a single exponentiation bounds the launch (Miller-Rabin bases on
+// a few survivors, serving iterations of wide classes).
+//
+// A number is Sp = Q*G radix-2^28 limbs; lane r of a group owns limbs
+// r*Q .. r*Q+Q-1 in registers.  Cross-lane traffic is DPP quad permutations
+// (no LDS): broadcasts inside the group, and moves to the next / previous
+// lane.  The Montgomery product is coarsely integrated operand scanning over
+// the group: for each limb a_i (broadcast from its owner) every lane adds
+// a_i*b_j and m_i*n_j into its 64-bit accumulator slots, the retired position
+// i leaves lane 0, and the window moves down one limb (the slot entering a
+// lane's top comes from the next lane).  Accumulator slots are not normalized
+// during the product: each position takes at most 2*Sp products below 2^56,
+// so Sp <= 127 keeps them below 2^64.  The result is then normalized with G-1
+// rounds of carry passing between the lanes.
+//
+// Values stay lazy as in the single-lane kernels (mont.hpp): R = 2^(28*Sp) >=
+// 16N, a product of inputs < 4N is < 2N.  The verdict logic is the one of
+// prp.hpp (miller_rabin.h:216-248), with group-wide reductions for the
+// comparisons.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+
+#include "prp.hpp"
+
+namespace pgx {
+namespace grp {
+
+constexpr int RB = 28;
+constexpr uint32_t MASK = (1u << RB) - 1u;
+
+constexpr int qp(int a, int b, int c, int d) { return a | (b << 2) | (c << 4) | (d << 6); }
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+// Value of group lane O (compile-time) in every lane of the group.
+template <int G, int O>
+__device__ __forceinline__ uint32_t bcast(uint32_t v) {
+  static_assert(G == 2 || G == 4, "groups of 2 or 4 lanes");
+  if constexpr (G == 4) return dpp<qp(O, O, O, O)>(v);
+  else return dpp<qp(O, O, 2 + O, 2 + O)>(v);
+}
+
+// Value of lane r+1 (the top lane gets its own value back: callers mask it).
+template <int G>
+__device__ __forceinline__ uint32_t from_next(uint32_t v) {
+  if constexpr (G == 4) return dpp<qp(1, 2, 3, 3)>(v);
+  else return dpp<qp(1, 1, 3, 3)>(v);
+}
+
+// Value of lane r-1 (lane 0 gets its own value back: callers mask it).
+template <int G>
+__device__ __forceinline__ uint32_t from_prev(uint32_t v) {
+  if constexpr (G == 4) return dpp<qp(0, 0, 1, 2)>(v);
+  else return dpp<qp(0, 0, 2, 2)>(v);
+}
+
+template <int G>
+__device__ __forceinline__ uint64_t from_next64(uint64_t v) {
+  return (uint64_t)from_next<G>((uint32_t)v) | ((uint64_t)from_next<G>((uint32_t)(v >> 32)) << 32);
+}
+
+template <int G>
+__device__ __forceinline__ int64_t from_prev64(int64_t v) {
+  const uint64_t u = (uint64_t)v;
+  return (int64_t)((uint64_t)from_prev<G>((uint32_t)u) |
+                   ((uint64_t)from_prev<G>((uint32_t)(u >> 32)) << 32));
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+  return (uint64_t)dpp<CTRL>((uint32_t)v) | ((uint64_t)dpp<CTRL>((uint32_t)(v >> 32)) << 32);
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t group_or(uint32_t v) {
+  v |= dpp<qp(1, 0, 3, 2)>(v);
+  if constexpr (G == 4) v |= dpp<qp(2, 3, 0, 1)>(v);
+  return v;
+}
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+  v += __longlong_as_double(
+      (long long)dpp64<qp(1, 0, 3, 2)>((uint64_t)__double_as_longlong(v)));
+  if constexpr (G == 4)
+    v += __longlong_as_double(
+        (long long)dpp64<qp(2, 3, 0, 1)>((uint64_t)__double_as_longlong(v)));
+  return v;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t lane_id() {
+  return threadIdx.x & (G - 1);
+}
+
+// Signed limbs t (any int64) -> normalized limbs r, carries passed through the
+// group; returns the carry out of the whole number (in every lane).
+template <int Q, int G>
+__device__ __forceinline__ int64_t settle(uint32_t (&r)[Q], const int64_t (&t)[Q]) {
+  const uint32_t lane = lane_id<G>();
+  int64_t c = 0;
+#pragma unroll
+  for (int s = 0; s < Q; ++s) {
+    const int64_t v = t[s] + c;
+    r[s] = (uint32_t)v & MASK;
+    c = v >> RB;
+  }
+  int64_t top = c;  // carries leaving this lane, summed over the rounds
+#pragma unroll
+  for (int round = 1; round < G; ++round) {
+    int64_t in = from_prev64<G>(c);
+    if (lane == 0) in = 0;
+#pragma unroll
+    for (int s = 0; s < Q; ++s) {
+      const int64_t v = (int64_t)r[s] + in;
+      r[s] = (uint32_t)v & MASK;
+      in = v >> RB;
+    }
+    c = in;
+    top += c;
+  }
+  return (int64_t)((uint64_t)bcast<G, G - 1>((uint32_t)top) |
+                   ((uint64_t)bcast<G, G - 1>((uint32_t)((uint64_t)top >> 32)) << 32));
+}
+
+// Unsigned accumulator slots -> normalized limbs (the carry out is zero for
+// every value this file produces).
+template <int Q, int G>
+__device__ __forceinline__ void settle_u(uint32_t (&r)[Q], const uint64_t (&t)[Q]) {
+  const uint32_t lane = lane_id<G>();
+  uint64_t c = 0;
+#pragma unroll
+  for (int s = 0; s < Q; ++s) {
+    const uint64_t v = t[s] + c;
+    r[s] = (uint32_t)v & MASK;
+    c = v >> RB;
+  }
+#pragma unroll
+  for (int round = 1; round < G; ++round) {
+    uint64_t in = (uint64_t)from_prev64<G>((int64_t)c);
+    if (lane == 0) in = 0;
+#pragma unroll
+    for (int s = 0; s < Q; ++s) {
+      const uint64_t v = (uint64_t)r[s] + in;
+      r[s] = (uint32_t)v & MASK;
+      in = v >> RB;
+    }
+    c = in;
+  }
+}
+
+// One position of the product scan: add a_I * b and m_I * n, retire
+// position I from lane 0, move the window down one limb.  I is a template
+// argument so the owner lane of a_I and its slot are constants.
+template <int Q, int G, int I>
+__device__ __forceinline__ void scan_step(uint64_t (&acc)[Q], const uint32_t (&a)[Q],
+                                          const uint32_t (&b)[Q], const uint32_t (&n)[Q],
+                                          uint32_t ninv, uint32_t lane) {
+  const uint32_t ai = bcast<G, I / Q>(a[I % Q]);
+#pragma unroll
+  for (int s = 0; s < Q; ++s) acc[s] += (uint64_t)ai * b[s];
+  const uint32_t m = bcast<G, 0>(((uint32_t)acc[0] * ninv) & MASK);
+#pragma unroll
+  for (int s = 0; s < Q; ++s) acc[s] += (uint64_t)m * n[s];
+  const uint64_t c0 = acc[0] >> RB;  // lane 0: the retired position's carry
+  uint64_t nx = from_next64<G>(acc[0]);
+  if (lane == G - 1) nx = 0;
+#pragma unroll
+  for (int s = 0; s + 1 < Q; ++s) acc[s] = acc[s + 1];
+  acc[Q - 1] = nx;
+  if (lane == 0) acc[0] += c0;
+}
+
+template <int Q, int G, int... Is>
+__device__ __forceinline__ void scan_all(std::integer_sequence<int, Is...>, uint64_t (&acc)[Q],
+                                         const uint32_t (&a)[Q], const uint32_t (&b)[Q],
+                                         const uint32_t (&n)[Q], uint32_t ninv, uint32_t lane) {
+  (scan_step<Q, G, Is>(acc, a, b, n, ninv, lane), ...);
+}
+
+// r <- a * b / R mod N (a, b < 4N with normalized limbs; r < 2N).  r may
+// alias a or b.
+template <int Q, int G>
+__device__ __forceinline__ void mont_mul(uint32_t (&r)[Q], const uint32_t (&a)[Q],
+                                         const uint32_t (&b)[Q], const uint32_t (&n)[Q],
+                                         uint32_t ninv) {
+  uint64_t acc[Q];
+#pragma unroll
+  for (int s = 0; s < Q; ++s) acc[s] = 0;
+  scan_all<Q, G>(std::make_integer_sequence<int, Q * G>{}, acc, a, b, n, ninv, lane_id<G>());
+  settle_u<Q, G>(r, acc);
+}
+
+// x <- 2x for bit == 1 (lazy: x < 2N becomes < 4N).
+template <int Q, int G>
+__device__ __forceinline__ void dbl_lazy(uint32_t (&x)[Q], uint32_t bit) {
+  const uint32_t lane = lane_id<G>();
+  uint32_t in = from_prev<G>(x[Q - 1] >> (RB - 1));
+  if (lane == 0) in = 0;
+  in &= bit;
+#pragma unroll
+  for (int s = Q - 1; s >= 1; --s) x[s] = ((x[s] << bit) | (x[s - 1] >> (RB - bit))) & MASK;
+  x[0] = ((x[0] << bit) & MASK) | in;
+}
+
+// x <- x - N when x >= N.
+template <int Q, int G>
+__device__ __forceinline__ void sub_if_ge(uint32_t (&x)[Q], const uint32_t (&n)[Q]) {
+  int64_t t[Q];
+#pragma unroll
+  for (int s = 0; s < Q; ++s) t[s] = (int64_t)x[s] - (int64_t)n[s];
+  uint32_t d[Q];
+  const bool lt = settle<Q, G>(d, t) < 0;
+#pragma unroll
+  for (int s = 0; s < Q; ++s) x[s] = lt ? x[s] : d[s];
+}
+
+template <int Q, int G>
+__device__ __forceinline__ void canon(uint32_t (&x)[Q], const uint32_t (&n)[Q]) {
+  sub_if_ge<Q, G>(x, n);
+  sub_if_ge<Q, G>(x, n);
+  sub_if_ge<Q, G>(x, n);
+}
+
+template <int Q, int G>
+__device__ __forceinline__ bool equal(const uint32_t (&a)[Q], const uint32_t (&b)[Q]) {
+  uint32_t diff = 0;
+#pragma unroll
+  for (int s = 0; s < Q; ++s) diff |= a[s] ^ b[s];
+  return group_or<G>(diff) == 0;
+}
+
+// a + b == N (canonical a, b).
+template <int Q, int G>
+__device__ __forceinline__ bool is_neg(const uint32_t (&a)[Q], const uint32_t (&b)[Q],
+                                       const uint32_t (&n)[Q]) {
+  int64_t t[Q];
+#pragma unroll
+  for (int s = 0; s < Q; ++s) t[s] = (int64_t)a[s] + (int64_t)b[s];
+  uint32_t sum[Q];
+  const int64_t c = settle<Q, G>(sum, t);
+  return c == 0 && equal<Q, G>(sum, n);
+}
+
+// v <- 2^bit (bit < 28*Sp).
+template <int Q, int G>
+__device__ __forceinline__ void set_pow2(uint32_t (&v)[Q], int bit) {
+  const int q = bit / RB - (int)lane_id<G>() * Q, rr = bit % RB;
+#pragma unroll
+  for (int s = 0; s < Q; ++s) v[s] = (s == q) ? (1u << rr) : 0u;
+}
+
+// one <- R mod N by doubling 2^(L-1) (< N).
+template <int Q, int G>
+__device__ __forceinline__ void r_mod_n(uint32_t (&one)[Q], const uint32_t (&n)[Q], int L) {
+  set_pow2<Q, G>(one, L - 1);
+  const int steps = RB * Q * G - (L - 1);
+  for (int c = 0; c < steps; ++c) {
+    dbl_lazy<Q, G>(one, 1u);
+    sub_if_ge<Q, G>(one, n);
+  }
+}
+
+// Sum of v_j 2^(28 j - L) over the group's limbs, as a double.
+template <int Q, int G>
+__device__ __forceinline__ double scaled(const uint32_t (&v)[Q], int L) {
+  const int base = (int)lane_id<G>() * Q;
+  double d = 0.0;
+#pragma unroll
+  for (int s = 0; s < Q; ++s) d += ldexp((double)v[s], RB * (base + s) - L);
+  return group_sum<G>(d);
+}
+
+// x <- a x mod N for an ordinary integer 2 <= a < 2^8 (x < 4N in, < 2N out);
+// the quotient estimate of mont.hpp's mul_small, summed over the group.
+template <int Q, int G>
+__device__ __forceinline__ void mul_small(uint32_t (&x)[Q], uint32_t a, const uint32_t (&n)[Q],
+                                          double inv_n, int L) {
+  const double qd = (double)a * scaled<Q, G>(x, L) * inv_n * (1.0 - 0x1p-40);
+  const uint32_t q = (qd > 0.0) ? (uint32_t)qd : 0u;
+  int64_t t[Q];
+#pragma unroll
+  for (int s = 0; s < Q; ++s)
+    t[s] = (int64_t)((uint64_t)x[s] * a) - (int64_t)((uint64_t)n[s] * q);
+  settle<Q, G>(x, t);
+}
+
+// Exponentiation of x = Mont(base) by N-1 with the strong-test checks (the
+// group form of powm_verdict in prp.hpp).  res <- base^(N-1) mod N.
+template <int Q, int G, class WordFn>
+__device__ __forceinline__ uint32_t powm_verdict(uint32_t (&x)[Q], const uint32_t (&n)[Q],
+                                                 uint32_t ninv, int L, int t, uint32_t base,
+                                                 double inv_n, WordFn word, uint32_t (&res)[Q]) {
+  uint32_t one[Q];
+  bool sprp = false;
+  if (t == L - 1) {
+    r_mod_n<Q, G>(one, n, L);
+    canon<Q, G>(x, n);
+    sprp = equal<Q, G>(x, one) || is_neg<Q, G>(x, one, n);
+  }
+  uint32_t w = 0;
+  for (int p = L - 2; p >= 0; --p) {
+    if (p == L - 2 || (p & 31) == 31) w = word(p >> 5);
+    const uint32_t bit = (p > 0) ? ((w >> (p & 31)) & 1u) : 0u;
+    mont_mul<Q, G>(x, x, x, n, ninv);
+    if (base == 2u) dbl_lazy<Q, G>(x, bit);
+    else if (bit) mul_small<Q, G>(x, base, n, inv_n, L);
+    if (p <= t && p >= 1) {
+      if (p == t) r_mod_n<Q, G>(one, n, L);
+      canon<Q, G>(x, n);
+      sprp = sprp || is_neg<Q, G>(x, one, n) || (p == t && equal<Q, G>(x, one));
+    }
+  }
+  uint32_t unit[Q];
+#pragma unroll
+  for (int s = 0; s < Q; ++s) unit[s] = (s == 0 && lane_id<G>() == 0) ? 1u : 0u;
+  mont_mul<Q, G>(res, x, unit, n, ninv);
+  sub_if_ge<Q, G>(res, n);
+  uint32_t one_diff = 0, m1_diff = 0;
+#pragma unroll
+  for (int s = 0; s < Q; ++s) {
+    const uint32_t nm1 = (s == 0 && lane_id<G>() == 0) ? n[s] - 1u : n[s];
+    one_diff |= res[s] ^ unit[s];
+    m1_diff |= res[s] ^ nm1;
+  }
+  const bool is_one = group_or<G>(one_diff) == 0, is_m1 = group_or<G>(m1_diff) == 0;
+  uint32_t f = 0;
+  if (is_one) f |= FLAG_FERMAT;
+  if (sprp) f |= FLAG_SPRP;
+  if (sprp || (t >= 2 && is_m1)) f |= FLAG_QUIRK;
+  return f;
+}
+
+}  // namespace grp
+}  // namespace pgx

@@ -30,11 +30,12 @@ int check_tuning(pgx_runner* h, const pgx_tuning_t& t) {
       t.blocks_per_cu > 64 || t.chunk < 64 || t.chunk % 64 || t.chunk > 4096)
     return fail(h, PGX_E_ARG, "bad tuning: blocks_per_cu=%u waves_per_block=%u chunk=%u",
                 t.blocks_per_cu, t.waves_per_block, t.chunk);
-  if (t.lanes_per_number != 1)
-    return fail(h, PGX_E_ARG, "lanes_per_number=%u: only one lane per candidate is built",
-                t.lanes_per_number);
+  if (t.lanes_per_number != 1 && t.lanes_per_number != 2 && t.lanes_per_number != 4)
+    return fail(h, PGX_E_ARG, "lanes_per_number=%u: 1, 2 or 4", t.lanes_per_number);
   return PGX_OK;
 }
+
+
 
 // Validates a bucket; returns the kernel limb count (> 0) or a negative code.
 int check_bucket(pgx_runner* h, const pgx_bucket_t* b, uint32_t flags) {
@@ -74,6 +75,20 @@ pgx::PersistShape shape_of(pgx_runner* h, const pgx_tuning_t& t) {
   return sh;
 }
 
+// Kernel of a bucket under a tuning: one lane per candidate (persistent), or a
+// group of lanes per candidate (one work item per group, plain grid; the
+// counter is unused).
+hipError_t launch_tuned(pgx_runner* h, int S, const pgx_tuning_t& t, const pgx::PrpArgs& a,
+                        uint32_t max_bits, uint32_t* counter, hipStream_t s) {
+  if (t.lanes_per_number == 1) {
+    hipError_t e = pgx::launch_fill(counter, 1, 0u, s);
+    return e != hipSuccess ? e : pgx::persist_table()[S](a, counter, shape_of(h, t), s);
+  }
+  const int Sp = pgx::group_width_for_bits(max_bits, (int)t.lanes_per_number);
+  if (Sp == 0) return hipErrorInvalidValue;
+  return pgx::group_table((int)t.lanes_per_number)[Sp](a, s);
+}
+
 pgx_tuning_t tuning_for(pgx_runner* h, int S, uint32_t flags, const pgx_tuning_t* given) {
   if (given) return *given;
   auto it = h->tuned.find(tune_key(S, flags));
@@ -97,13 +112,14 @@ int launch_persistent(pgx_runner* h, const pgx_bucket_t* b, const pgx_tuning_t* 
   int rc = check_tuning(h, t);
   if (rc != PGX_OK) return rc;
   if (b->n == 0) return PGX_OK;
+  if (t.lanes_per_number > 1 && !pgx::group_width_for_bits(b->max_bits, (int)t.lanes_per_number))
+    return fail(h, PGX_E_INPUT, "no %u-lane kernel holds %u bits", t.lanes_per_number,
+                b->max_bits);
   pgx::PrpArgs a = args_of(h, b, flags);
   a.index = d_index;
   a.count = d_count;
-  uint32_t* counter = next_counter(h);
   PGX_HIP(h, hipSetDevice(h->device));
-  PGX_HIP(h, pgx::launch_fill(counter, 1, 0u, s));
-  PGX_HIP(h, pgx::persist_table()[S](a, counter, shape_of(h, t), s));
+  PGX_HIP(h, launch_tuned(h, S, t, a, b->max_bits, next_counter(h), s));
   return PGX_OK;
 }
 
@@ -197,6 +213,27 @@ int pgx_autotune(pgx_runner_t* h, const pgx_bucket_t* b, uint32_t flags, pgx_tun
           top = t;
         }
       }
+  for (uint32_t lanes : {2u, 4u}) {  // a group of lanes per candidate
+    if (rc != PGX_OK) break;
+    if (!pgx::group_width_for_bits(b->max_bits, (int)lanes)) continue;
+    pgx_tuning_t t = default_tuning();
+    t.lanes_per_number = lanes;
+    float best_ms = 0;
+    for (int rep = 0; rep < 2 && rc == PGX_OK; ++rep) {
+      hipError_t e = hipEventRecord(e0, h->stream);
+      if (e == hipSuccess) rc = launch_persistent(h, &sample, &t, flags, nullptr, nullptr, h->stream);
+      if (e == hipSuccess && rc == PGX_OK) e = hipEventRecord(e1, h->stream);
+      if (e == hipSuccess && rc == PGX_OK) e = hipEventSynchronize(e1);
+      float ms = 0;
+      if (e == hipSuccess && rc == PGX_OK) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) rc = fail(h, PGX_E_HIP, "autotune timing: %s", hipGetErrorString(e));
+      if (rc == PGX_OK && (best_ms == 0 || ms < best_ms)) best_ms = ms;
+    }
+    if (rc == PGX_OK && best_ms < top_ms) {
+      top_ms = best_ms;
+      top = t;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != PGX_OK) return rc;
@@ -220,6 +257,10 @@ int pgx_plan_create(pgx_plan_t** out, pgx_runner_t* h, const pgx_bucket_t* bucke
     T[k] = tuning_for(h, S[k], flags, tuning ? &tuning[k] : nullptr);
     int rc = check_tuning(h, T[k]);
     if (rc != PGX_OK) return rc;
+    if (T[k].lanes_per_number > 1 &&
+        !pgx::group_width_for_bits(buckets[k].max_bits, (int)T[k].lanes_per_number))
+      return fail(h, PGX_E_INPUT, "no %u-lane kernel holds %u bits", T[k].lanes_per_number,
+                  buckets[k].max_bits);
   }
   PGX_HIP(h, hipSetDevice(h->device));
   pgx_plan* p = new (std::nothrow) pgx_plan();
@@ -248,10 +289,9 @@ int pgx_plan_create(pgx_plan_t** out, pgx_runner_t* h, const pgx_bucket_t* bucke
   for (uint32_t k = 0; k < nbuckets && e == hipSuccess; ++k) {
     hipStream_t s = p->streams[k + 1];
     e = hipStreamWaitEvent(s, p->events[0], 0);
-    if (e == hipSuccess) e = pgx::launch_fill(p->d_counters + k, 1, 0u, s);
     if (e == hipSuccess && buckets[k].n > 0) {
       pgx::PrpArgs a = args_of(h, &buckets[k], flags);
-      e = pgx::persist_table()[S[k]](a, p->d_counters + k, shape_of(h, T[k]), s);
+      e = launch_tuned(h, S[k], T[k], a, buckets[k].max_bits, p->d_counters + k, s);
     }
     if (e == hipSuccess) e = hipEventRecord(p->events[k + 1], s);
     if (e == hipSuccess) e = hipStreamWaitEvent(cap, p->events[k + 1], 0);

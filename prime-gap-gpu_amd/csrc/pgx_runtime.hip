@@ -25,6 +25,12 @@ PersistLaunchFn* persist_table() {
   return table;
 }
 
+PrpLaunchFn* group_table(int G) {
+  static PrpLaunchFn two[kMaxKernelLimbs + 1] = {};
+  static PrpLaunchFn four[kMaxKernelLimbs + 1] = {};
+  return G == 2 ? two : four;
+}
+
 std::string& create_error() {
   static std::string msg = "no error";
   return msg;

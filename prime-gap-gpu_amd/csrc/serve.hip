@@ -246,6 +246,13 @@ int launch_iteration(pgx_server* s, Class& c, bool copies_pending) {
   }
   PGX_HIP(s->h, hipMemcpyAsync(c.d_index, c.h_index, (size_t)items * 4, hipMemcpyHostToDevice,
                                c.stream));
+  // a small iteration of a wide class: a group of lanes per candidate
+  const uint32_t cap_bits = (uint32_t)pgx::kernel_capacity_bits(c.S);
+  const int G = (int)s->cfg.group_lanes;
+  const int Sp = (s->cfg.group_max_items && items <= s->cfg.group_max_items &&
+                  cap_bits >= s->cfg.group_min_bits)
+                     ? pgx::group_width_for_bits(cap_bits, G)
+                     : 0;
   pgx::PrpArgs a{};
   a.cand = c.d_pool;
   a.ld = c.cap;
@@ -255,7 +262,7 @@ int launch_iteration(pgx_server* s, Class& c, bool copies_pending) {
   a.bases = s->h->bases;  // {2}
   a.verdict = c.d_verdict;
   a.index = c.d_index;
-  PGX_HIP(s->h, pgx::prp_table()[c.S](a, c.stream));
+  PGX_HIP(s->h, (Sp ? pgx::group_table(G)[Sp] : pgx::prp_table()[c.S])(a, c.stream));
   PGX_HIP(s->h, pgx::launch_gather(c.d_verdict, c.d_index, items, c.d_out, c.stream));
   PGX_HIP(s->h, hipMemcpyAsync(c.h_out, c.d_out, (size_t)items * 4, hipMemcpyDeviceToHost,
                                c.stream));
@@ -329,7 +336,8 @@ int pgx_serve_create(pgx_server_t** out, int device, const pgx_serve_config_t* c
   if (!out || !cfg) return fail(nullptr, PGX_E_ARG, "null argument");
   *out = nullptr;
   if (cfg->chunk == 0 || cfg->max_requests == 0 || cfg->pool_columns == 0 ||
-      cfg->max_side == 0 || cfg->max_side * 2ull > cfg->pool_columns)
+      cfg->max_side == 0 || cfg->max_side * 2ull > cfg->pool_columns ||
+      (cfg->group_max_items && cfg->group_lanes != 2 && cfg->group_lanes != 4))
     return fail(nullptr, PGX_E_ARG, "bad serving config");
   const uint32_t base2 = 2;
   pgx_runner_t* h = nullptr;

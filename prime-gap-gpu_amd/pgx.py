@@ -110,7 +110,9 @@ class Bucket(ctypes.Structure):
 class ServeConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("chunk", ctypes.c_uint32),
                 ("max_requests", ctypes.c_uint32), ("pool_columns", ctypes.c_uint32),
-                ("max_side", ctypes.c_uint32), ("stage_columns", ctypes.c_uint32)]
+                ("max_side", ctypes.c_uint32), ("stage_columns", ctypes.c_uint32),
+                ("group_min_bits", ctypes.c_uint32), ("group_max_items", ctypes.c_uint32),
+                ("group_lanes", ctypes.c_uint32)]
 
 
 class ServeRequest(ctypes.Structure):
@@ -607,6 +609,9 @@ class DeviceBatch:
         self.verdict = _torch.zeros(max(self.n, 1), dtype=_torch.int32, device="cuda")
         self.res = (_torch.zeros((self.limbs, self.ld), dtype=_torch.int32, device="cuda")
                     if residues else None)
+        # the fills above run on torch's stream; launches on other streams
+        # (the runner's own, non-blocking) must not overtake them
+        _torch.cuda.synchronize()
 
     def bucket(self) -> Bucket:
         return Bucket(self.cand.data_ptr(), self.ld, self.limbs, self.n, self.max_bits,
@@ -625,8 +630,10 @@ class Server:
 
     def __init__(self, hbm_budget: int = 64 << 30, chunk: int = 32, max_requests: int = 8192,
                  pool_columns: int = 1 << 21, max_side: int = 512, stage_columns: int = 1 << 17,
+                 group_min_bits: int = 0, group_max_items: int = 0, group_lanes: int = 4,
                  device: int = 0):
-        cfg = ServeConfig(hbm_budget, chunk, max_requests, pool_columns, max_side, stage_columns)
+        cfg = ServeConfig(hbm_budget, chunk, max_requests, pool_columns, max_side, stage_columns,
+                          group_min_bits, group_max_items, group_lanes)
         h = ctypes.c_void_p()
         rc = lib.pgx_serve_create(ctypes.byref(h), device, ctypes.byref(cfg))
         if rc != PGX_OK:

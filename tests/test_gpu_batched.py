@@ -148,7 +148,7 @@ def test_bad_tuning_and_bucket_are_rejected():
     L = gen(337, 2310, 16, 1, 64)
     b = pgx.DeviceBatch(L)
     with pgx.Runner(1, limbs=16) as run:
-        for bad in (pgx.Tuning(1, 1, 100, 1), pgx.Tuning(1, 5, 64, 1), pgx.Tuning(1, 1, 64, 4)):
+        for bad in (pgx.Tuning(1, 1, 100, 1), pgx.Tuning(1, 5, 64, 1), pgx.Tuning(1, 1, 64, 3)):
             with pytest.raises(pgx.PgxError) as e:
                 run.test_persistent(b.bucket(), pgx.MODE_FERMAT, tuning=bad)
             assert e.value.code == pgx.PGX_E_ARG

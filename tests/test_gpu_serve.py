@@ -60,8 +60,11 @@ def drain(srv, want, timeout=60.0):
     return got
 
 
-def test_alone_and_mixed_equal_oracle(requests, expected):
-    with pgx.Server(chunk=16) as srv:
+@pytest.mark.parametrize("group", [(0, 0, 4), (400, 1 << 20, 4), (400, 1 << 20, 2)])
+def test_alone_and_mixed_equal_oracle(requests, expected, group):
+    """Also with the wide classes' iterations on 2- or 4-lane groups."""
+    gb, gi, gl = group
+    with pgx.Server(chunk=16, group_min_bits=gb, group_max_items=gi, group_lanes=gl) as srv:
         alone = {}
         for i, (P, words, nxt, prv) in enumerate(requests):
             srv.submit(i, nxt, prv, bits(P))
@@ -98,6 +101,8 @@ def test_budget_forces_queueing(requests, expected):
 
 
 def test_rejections_and_empty_request(requests):
+    with pytest.raises(pgx.PgxError):
+        pgx.Server(group_max_items=64, group_lanes=3)
     P, words, nxt, prv = requests[0]
     with pgx.Server(hbm_budget=1000, max_side=512) as srv:
         with pytest.raises(pgx.PgxError) as e:  # needs more than the budget
