@@ -183,7 +183,9 @@ typedef struct {
   uint32_t blocks_per_cu;    /* resident workgroups per compute unit */
   uint32_t waves_per_block;  /* wavefronts per workgroup, 1..4 */
   uint32_t chunk;            /* candidates a wavefront takes per counter bump, multiple of 64 */
-  uint32_t lanes_per_number; /* lanes that share one candidate; this build has 1 */
+  uint32_t lanes_per_number; /* lanes that share one candidate: 1 (persistent workgroups,
+                                the fields above apply), 2 or 4 (a lane group per candidate,
+                                one work item per group: for launches with few candidates) */
 } pgx_tuning_t;
 
 typedef struct pgx_plan pgx_plan_t;
