@@ -199,6 +199,44 @@ def bench_persistent(cands: np.ndarray, stream, peak: float, bits: int, reps: in
     }
 
 
+def bench_1024(stream, peak: float, reps: int) -> dict:
+    """The 1024-bit class alone: P=701, d=210 (987-bit N in 32-word buffers),
+    2^18 candidates; plain launches of 65536 vs one autotuned launch."""
+    import pgx
+    L = gen(701, 210, 32, 16, 16384, SEED + 701)
+    a = pgx.DeviceBatch(L)
+    run = pgx.Runner(1, limbs=32)
+    t, _ = run.autotune(a.bucket(), pgx.MODE_FERMAT)
+
+    def plain():
+        for c0 in range(0, a.n, BATCH):
+            run.test_device(a.cand.data_ptr() + 4 * c0, a.ld, a.limbs, min(BATCH, a.n - c0),
+                            a.max_bits, pgx.MODE_FERMAT, a.verdict.data_ptr() + 4 * c0,
+                            a.res.data_ptr() + 4 * c0, a.ld, stream.cuda_stream)
+
+    def tuned():
+        run.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True, tuning=t,
+                            stream=stream.cuda_stream)
+    plain()
+    ms_plain = time_ms(stream, plain, reps)
+    tuned()
+    ms_tuned = time_ms(stream, tuned, reps)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    ov, ores = oracle.verdicts(L[:1024], oracle.MODE_FERMAT, residues=True)
+    ok = bool(np.array_equal(a.verdicts()[:1024], ov) and np.array_equal(a.residues()[:1024], ores))
+    run.close()
+    bits = median_bits(L)
+    w = work_per_test(bits) * a.n
+    return {
+        "workload": "P=701 d=210, 2^18 sieved candidates (~987 bits), 32-word buffers, Fermat",
+        "bits": bits, "tuned": t.as_dict(),
+        "plain_tests_per_s": a.n / (ms_plain * 1e-3), "plain_valu_frac": w / (ms_plain * 1e-3) / peak,
+        "tuned_tests_per_s": a.n / (ms_tuned * 1e-3), "tuned_valu_frac": w / (ms_tuned * 1e-3) / peak,
+        "tuned_over_plain": ms_plain / ms_tuned, "parity_sample_ok": ok,
+    }
+
+
 def bench_large(stream, peak: float, reps: int) -> dict:
     """P=907, d=30030 (~1260-bit N in 64-word buffers), 2^18 candidates:
     base-2 Fermat (persistent, autotuned), on-device compaction of the PRPs,
@@ -487,7 +525,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline plain-kernel workload")
-    ap.add_argument("--extras", default="persistent,mixed,large,search,serving",
+    ap.add_argument("--extras", default="persistent,1024,mixed,large,search,serving",
                     help="comma-separated extra configurations to run")
     ap.add_argument("--serve-group", default="1000,32768,4",
                     help="bits,items,lanes: serving iterations of classes at least `bits` wide "
@@ -596,6 +634,10 @@ def main() -> int:
         extra["same_workload_persistent"] = bench_persistent(cands, stream, peak, L,
                                                              max(2, args.steps // 4))
         log(f"persistent: {extra['same_workload_persistent']['tests_per_s'] / 1e6:.2f} M tests/s")
+    if "1024" in todo:
+        extra["class_1024"] = bench_1024(stream, peak, max(2, args.steps // 4))
+        log(f"1024-bit: plain {extra['class_1024']['plain_tests_per_s'] / 1e6:.2f} M/s, "
+            f"tuned {extra['class_1024']['tuned_tests_per_s'] / 1e6:.2f} M/s")
     if "mixed" in todo:
         extra["mixed_graph"] = bench_mixed(stream, peak, max(2, args.steps // 4))
     if "large" in todo:

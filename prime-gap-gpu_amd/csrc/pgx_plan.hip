@@ -125,6 +125,19 @@ int launch_persistent(pgx_runner* h, const pgx_bucket_t* b, const pgx_tuning_t* 
 
 }  // namespace
 
+int pgx::launch_auto(pgx_runner* h, const PrpArgs& a, uint32_t max_bits, hipStream_t s) {
+  const int S = pgx::kernel_for_bits(max_bits);
+  if (S == 0 || !pgx::persist_table()[S])
+    return fail(h, PGX_E_INPUT, "candidates of %u bits exceed the widest kernel (%d bits)",
+                max_bits, PGX_MAX_BITS);
+  const pgx_tuning_t t = tuning_for(h, S, a.mode, nullptr);
+  if (t.lanes_per_number > 1 && !pgx::group_width_for_bits(max_bits, (int)t.lanes_per_number))
+    return fail(h, PGX_E_INPUT, "no %u-lane kernel holds %u bits", t.lanes_per_number, max_bits);
+  if (a.n == 0) return PGX_OK;
+  PGX_HIP(h, launch_tuned(h, S, t, a, max_bits, next_counter(h), s));
+  return PGX_OK;
+}
+
 struct pgx_plan {
   pgx_runner* h = nullptr;
   hipGraph_t graph = nullptr;

@@ -75,6 +75,10 @@ int launch_prp(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limb
                uint32_t max_bits, uint32_t flags, uint32_t* d_verdict, uint32_t* d_res,
                uint32_t res_ld, hipStream_t stream);
 
+// Launch over device buffers with the runner's tuned (or default) shape for
+// this width and mode: persistent one-lane workgroups or lane groups.
+int launch_auto(pgx_runner* h, const PrpArgs& a, uint32_t max_bits, hipStream_t stream);
+
 // Column i of out (out_words rows, leading dimension ld) = m_i * K + sign * x_i
 // with m_i = m64[i] (m64 != NULL) or m_base + m_off[i], and x_i = x[i] (x != NULL)
 // or x0.  *status |= 1 when a column does not fit out_words or is negative.

@@ -610,8 +610,15 @@ int pgx_search_test(pgx_search_t* s, uint32_t* n_found) {
     PGX_SHIP(s, pgx::launch_build(s->d_K, s->k_words, nullptr, s->d_tests + off,
                                   s->cfg.m_start, nullptr, s->cur_x, 1, nb, h->d_cand, nb,
                                   h->limbs, s->d_words + 2, st));
-    int rc = pgx::launch_prp(h, h->d_cand, nb, h->limbs, nb, s->max_bits, PGX_MODE_REFERENCE,
-                             h->d_verdict, nullptr, nb, st);
+    pgx::PrpArgs pa{};
+    pa.cand = h->d_cand;
+    pa.ld = nb;
+    pa.n = nb;
+    pa.buf_limbs = h->limbs;
+    pa.mode = PGX_MODE_REFERENCE;
+    pa.bases = h->bases;
+    pa.verdict = h->d_verdict;
+    int rc = pgx::launch_auto(h, pa, s->max_bits, st);
     if (rc != PGX_OK) return sfail(s, rc, h->err.c_str());
     hipLaunchKernelGGL(k_consume, dim3((nb + kConsumeBlock - 1) / kConsumeBlock),
                        dim3(kConsumeBlock), 0, st,
