@@ -386,3 +386,37 @@ double orc_time_powm2(const uint32_t* limbs, size_t n, uint32_t S, int threads, 
   if (fermat_count) *fermat_count = ones;
   return t1 - t0;
 }
+
+/* ---- overflow search (overflow.cpp:142-160) ------------------------------ */
+
+/* Offset X of the next prime above m*K (sign > 0: mpz_nextprime(m*K), the
+ * reference's next side with min_x = 0) or of the previous prime below it
+ * (sign < 0: the reference calls mpz_prevprime, which GMP 6.2.1 lacks; here
+ * odd y = m*K - 1, m*K - 3, ... with mpz_probab_prime_p(y, 25), the test GMP's
+ * own search uses).  *x = 0 when the offset exceeds limit. */
+int orc_next_prime_offset(const uint32_t* K, uint32_t k_words, uint64_t m, int sign,
+                          uint32_t limit, uint32_t* x) {
+  if (!K || !k_words || !x || (sign != 1 && sign != -1)) return ORC_E_ARG;
+  mpz_t k, c, y;
+  mpz_inits(k, c, y, NULL);
+  to_mpz(k, K, k_words);
+  mpz_mul_ui(c, k, (unsigned long)m);
+  *x = 0;
+  if (sign > 0) {
+    mpz_nextprime(y, c);
+    mpz_sub(y, y, c);
+    if (mpz_cmp_ui(y, limit) <= 0) *x = (uint32_t)mpz_get_ui(y);
+  } else {
+    mpz_sub_ui(y, c, mpz_odd_p(c) ? 2 : 1);
+    for (uint32_t off = mpz_odd_p(c) ? 2 : 1; off <= limit; off += 2) {
+      if (mpz_probab_prime_p(y, 25)) {
+        *x = off;
+        break;
+      }
+      mpz_sub_ui(y, y, 2);
+    }
+  }
+  mpz_clears(k, c, y, NULL);
+  return ORC_OK;
+}
+

@@ -11,6 +11,7 @@ Reference functions followed (file:line in sethtroisi/prime-gap-gpu):
   residue          BASELINE CPU path: mpz_powm(2, N-1, N)
   gen_*            gap_common.cpp:60-65 (K = P#/d), gpu_testing.cu:171-172 (N = m*K + x),
                    gap_search_gpu.cpp:619-689 (sieve of m*K + X by small primes)
+  next_prime_offset overflow.cpp:142-160 (mpz_nextprime / previous prime around m*K)
   ms_*             the m-interval search step (msieve.c): is_coprime_and_valid_m,
                    coprime_X, run_sieve_thread's composites bitmap, finalize +
                    fill_batch + process_finished_batch + remove_vector
@@ -52,6 +53,8 @@ def _load():
     lib.orc_time_powm2.argtypes = [u32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int,
                                    ctypes.c_int, u32p]
     lib.orc_time_powm2.restype = ctypes.c_double
+    lib.orc_next_prime_offset.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                                          ctypes.c_uint32, u32p]
     szp = ctypes.POINTER(ctypes.c_size_t)
     lib.orc_ms_active.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                   ctypes.c_uint32, u32p, szp]
@@ -142,6 +145,16 @@ def time_powm2(limbs: np.ndarray, threads: int, warmup: bool = True):
     cnt = np.zeros(1, dtype=np.uint32)
     sec = lib.orc_time_powm2(_u32(a), a.shape[0], a.shape[1], threads, int(warmup), _u32(cnt))
     return sec, int(cnt[0])
+
+
+def next_prime_offset(K_words, m: int, sign: int, limit: int) -> int:
+    """X of GMP's next prime above m*K (sign 1) or previous below (sign -1), 0 if > limit."""
+    k = np.ascontiguousarray(K_words, dtype=np.uint32)
+    out = np.zeros(1, dtype=np.uint32)
+    rc = lib.orc_next_prime_offset(_u32(k), len(k), m, sign, limit, _u32(out))
+    if rc != 0:
+        raise ValueError(f"orc_next_prime_offset failed: {rc}")
+    return int(out[0])
 
 
 # ---- m-interval search step (msieve.c) ------------------------------------

@@ -118,3 +118,30 @@ def test_rejections_and_empty_request(requests):
         srv.submit(4, nxt[:0], prv[:0], bits(P))
         got = drain(srv, 1)
         assert got[4][:2] == (-1, -1)
+
+
+def test_overflow_search_equals_gmp_nextprime():
+    """The reference's overflow step (overflow.cpp:142-160): the next prime above
+    m*K (mpz_nextprime) and the previous one below it.  Served over the sieved
+    candidates of each side, the first PRP's X equals GMP's answer."""
+    cases = []
+    for P in (337, 503, 907):
+        K = pgx.primorial_over(P, 2310)
+        kw = pgx.to_limbs([K], (K.bit_length() + 31) // 32)[0]
+        words = (K.bit_length() + 24 + 31) // 32 + 1
+        m = pgx.gen_multipliers(0x5E0A00 + P, 10**7, 10**7 + 10**4, 2310, 4)
+        sides = [pgx.gen_sieved(P, 2310, m, 30000, 512, words, sign=sg) for sg in (1, -1)]
+        for k, mm in enumerate(m):
+            n_l, p_l = (L[om == k] for L, om, _ in sides)
+            n_x, p_x = (ox[om == k] for _, om, ox in sides)
+            want = tuple(O.next_prime_offset(kw, int(mm), sg, 30000) for sg in (1, -1))
+            cases.append((P, n_l, p_l, n_x, p_x, want))
+    with pgx.Server(chunk=32, group_min_bits=1000, group_max_items=1 << 15, group_lanes=4) as srv:
+        for i, (P, n_l, p_l, n_x, p_x, want) in enumerate(cases):
+            srv.submit(i, n_l, p_l, bits(P))
+        got = drain(srv, len(cases))
+    for i, (P, n_l, p_l, n_x, p_x, want) in enumerate(cases):
+        nf, pf, _ = got[i]
+        have = (int(n_x[nf]) if nf >= 0 else 0, int(p_x[pf]) if pf >= 0 else 0)
+        say(f"P={P} m #{i}: next/prev gap offsets {have} (GMP {want})")
+        assert have == want, (P, i)

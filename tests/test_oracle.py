@@ -152,3 +152,15 @@ def test_cpu_baseline_timer_counts_fermat():
     sec, cnt = O.time_powm2(L, threads=2)
     f = O.verdicts(L, O.MODE_FERMAT)
     assert sec > 0 and cnt == int((f & 1).sum())
+
+
+def test_next_prime_offset_vs_sympy():
+    sympy = pytest.importorskip("sympy")
+    import pgx
+    K = pgx.primorial_over(337, 2310)
+    kw = pgx.to_limbs([K], (K.bit_length() + 31) // 32)[0]
+    for m in (10000019, 10000079):
+        c = m * K
+        assert O.next_prime_offset(kw, m, 1, 30000) == sympy.nextprime(c) - c
+        assert O.next_prime_offset(kw, m, -1, 30000) == c - sympy.prevprime(c)
+    assert O.next_prime_offset(kw, 10000019, 1, 10) == 0  # beyond the limit
