@@ -396,7 +396,7 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
     """Poisson stream of per-m next/prev-prime requests (P in {337, 503, 907},
     d=2310, 20-500 sieved X per side), continuous batching under a 64 GB cap.
     Each request is submitted at its scheduled arrival time; latency runs from
-    that time to completion, so driver lateness counts against the engine."""
+    that time to completion, so lateness of this load generator counts against the engine."""
     import oracle
     import pgx
     sizes = [(337, 16), (503, 24), (907, 48)]
@@ -434,7 +434,7 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
                 raise RuntimeError(f"serving warm-up did not finish: {srv.stats()}")
         import gc
         gc.collect()
-        gc.disable()  # a collector pause of this driver would count as engine latency
+        gc.disable()  # a collector pause of this loop would count as engine latency
         start = time.monotonic_ns()
         arrivals = start + np.cumsum(gaps * 1e9).astype(np.int64)
         i = 0
@@ -537,7 +537,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path "
-              "shards by candidate with no exchange; see DESIGN.md)", file=sys.stderr)
+              "shards by candidate with no exchange: one runner per device)", file=sys.stderr)
         return 2
 
     import torch

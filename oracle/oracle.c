@@ -5,7 +5,7 @@
  * product (libpgx.so) never calls it.
  *
  * Everything is computed with GMP (6.2.1 from /opt/conda in the build image):
- *   - residues a^(N-1) mod N with mpz_powm, as BASELINE's CPU path does;
+ *   - residues a^(N-1) mod N with mpz_powm (the CPU path timed beside the GPU);
  *   - the reference's verdict chain, restated step by step from
  *     miller_rabin_t::miller_rabin (miller_rabin.h:216-248): d = (N-1)>>t,
  *     x = a^d, early pass on {1, N-1}, immediate fail when t == 1, otherwise

@@ -8,7 +8,7 @@ small inputs (CPython's own bignum ``pow``, independent of GMP).
 Reference functions followed (file:line in sethtroisi/prime-gap-gpu):
   ref_verdict      miller_rabin.h:216-248  (strong test as written, incl. t squarings)
   sprp             textbook strong test (the reference's intent)
-  residue          BASELINE CPU path: mpz_powm(2, N-1, N)
+  residue          the CPU baseline path: mpz_powm(2, N-1, N)
   gen_*            gap_common.cpp:60-65 (K = P#/d), gpu_testing.cu:171-172 (N = m*K + x),
                    gap_search_gpu.cpp:619-689 (sieve of m*K + X by small primes)
   next_prime_offset overflow.cpp:142-160 (mpz_nextprime / previous prime around m*K)

@@ -184,3 +184,12 @@ def test_lane_routine_small_bases_tiny_n(emu):
     for S in (1, 2, 3, 10):
         v, _ = emu(L, S, pgx.MODE_REFERENCE, bases)
         assert v.tolist() == want, S
+
+
+def test_bench_refuses_several_gpus():
+    import subprocess
+    import sys as _sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([_sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
