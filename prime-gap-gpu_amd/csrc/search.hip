@@ -107,6 +107,95 @@ __device__ __forceinline__ uint32_t mod_u52(uint64_t v, uint32_t q) {
   return (uint32_t)r;
 }
 
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds,
+                                                         uint32_t* total) {
+  const uint32_t tid = threadIdx.x, n = blockDim.x;
+  lds[tid] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < n; off <<= 1) {
+    const uint32_t add = (tid >= off) ? lds[tid - off] : 0u;
+    __syncthreads();
+    lds[tid] += add;
+    __syncthreads();
+  }
+  const uint32_t incl = lds[tid];
+  *total = lds[n - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+// Primes in [lo, hi) (lo odd), one segment of kSegOdd odd numbers per
+// workgroup: the base primes (odd, up to sqrt(hi)) mark their odd multiples
+// in an LDS bitmap (one wavefront per base prime below 4096, one lane per
+// larger one), then the unmarked numbers are appended to out[] with one
+// atomic per workgroup.  The order of the list is not defined.
+constexpr uint32_t kSegWords = 1024;
+constexpr uint32_t kSegOdd = kSegWords * 32;
+
+__global__ __launch_bounds__(kBlock) void k_gen_primes(const uint32_t* __restrict__ base,
+                                                       uint32_t n_base, uint64_t lo, uint64_t hi,
+                                                       uint32_t* __restrict__ out,
+                                                       uint32_t* __restrict__ count) {
+  __shared__ uint32_t seg[kSegWords];
+  __shared__ uint32_t lds[kBlock];
+  __shared__ uint32_t block_base;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint64_t seg_lo = lo + 2ull * kSegOdd * blockIdx.x;  // odd
+  const uint64_t seg_hi = seg_lo + 2ull * kSegOdd;           // exclusive
+  for (uint32_t w = tid; w < kSegWords; w += kBlock) seg[w] = 0;
+  __syncthreads();
+  auto first_index = [&](uint32_t q) -> uint64_t {  // odd-index of the first odd multiple to mark
+    const uint64_t q2 = (uint64_t)q * q;
+    uint64_t st = (seg_lo + q - 1) / q * q;
+    if (st < q2) st = q2;
+    if (!(st & 1)) st += q;
+    return (st - seg_lo) >> 1;
+  };
+  for (uint32_t j = wave; j < n_base; j += kBlock / 64) {
+    const uint32_t q = base[j];
+    if (q >= 4096) break;  // the rest go one lane each
+    if ((uint64_t)q * q >= seg_hi) break;
+    for (uint64_t t = first_index(q) + (uint64_t)lane * q; t < kSegOdd; t += 64ull * q)
+      atomicOr(&seg[t >> 5], 1u << (t & 31u));
+  }
+  for (uint32_t j = tid; j < n_base; j += kBlock) {
+    const uint32_t q = base[j];
+    if (q < 4096) continue;
+    if ((uint64_t)q * q >= seg_hi) break;
+    for (uint64_t t = first_index(q); t < kSegOdd; t += q)
+      atomicOr(&seg[t >> 5], 1u << (t & 31u));
+  }
+  __syncthreads();
+  constexpr uint32_t kPer = kSegWords / kBlock;  // words per thread
+  uint32_t keep[kPer];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t w = tid * kPer + k;
+    uint32_t v = ~seg[w];
+    const uint64_t v0 = seg_lo + 64ull * w;  // value of bit 0 of word w
+    if (v0 >= hi) v = 0;
+    else if (hi - v0 < 64) v &= (1u << ((hi - v0 + 1) / 2)) - 1u;  // values below hi
+    keep[k] = v;
+    cnt += __popc(v);
+  }
+  uint32_t total;
+  uint32_t pos = block_exclusive_scan(cnt, lds, &total);
+  if (tid == 0) block_base = total ? atomicAdd(count, total) : 0u;
+  __syncthreads();
+  pos += block_base;
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; ++k) {
+    uint32_t v = keep[k];
+    const uint64_t v0 = seg_lo + 64ull * (tid * kPer + k);
+    while (v) {
+      const uint32_t b = __ffs(v) - 1u;
+      v &= v - 1u;
+      out[pos++] = (uint32_t)(v0 + 2ull * b);
+    }
+  }
+}
+
 // Smallest odd offset f with q | (m_start + f) K + X, halved: the first bit.
 __device__ __forceinline__ uint32_t first_bit(uint32_t q, uint32_t nik, uint32_t msm, uint32_t X) {
   const uint64_t v = (uint64_t)X * nik + (q - msm);  // < 2^52 for X < 2^20
@@ -211,23 +300,6 @@ __device__ __forceinline__ uint32_t select_word(const uint32_t* A, const uint32_
   if (C) v &= ~C[w];
   if (F) v &= ~F[w];
   return v;
-}
-
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds,
-                                                         uint32_t* total) {
-  const uint32_t tid = threadIdx.x, n = blockDim.x;
-  lds[tid] = v;
-  __syncthreads();
-  for (uint32_t off = 1; off < n; off <<= 1) {
-    const uint32_t add = (tid >= off) ? lds[tid - off] : 0u;
-    __syncthreads();
-    lds[tid] += add;
-    __syncthreads();
-  }
-  const uint32_t incl = lds[tid];
-  *total = lds[n - 1];
-  __syncthreads();
-  return incl - v;
 }
 
 __global__ __launch_bounds__(kBlock) void k_count(const uint32_t* __restrict__ A,
@@ -430,6 +502,7 @@ int pgx_search_create(pgx_search_t** out, pgx_runner_t* h, const uint32_t* K, ui
       (c.flags & ~PGX_SEARCH_AS_WRITTEN))
     return pgx::fail(h, PGX_E_ARG, "bad search configuration");
   if (!(K[0] & 1u)) return pgx::fail(h, PGX_E_ARG, "K must be odd");
+  if (c.P >= kSmallPrimeLimit) return pgx::fail(h, PGX_E_ARG, "P must be below 100000");
   uint32_t kb = 0;
   for (int j = (int)k_words - 1; j >= 0; --j)
     if (K[j]) {
@@ -456,9 +529,11 @@ int pgx_search_create(pgx_search_t** out, pgx_runner_t* h, const uint32_t* K, ui
   s->k_words = k_words;
   s->max_bits = max_bits;
 
-  // sieving primes: odd q < max_prime with q > P, or q | d (the d wheel)
-  std::vector<uint32_t> all, primes, dps;
-  odd_primes_below(c.max_prime, all);
+  // sieving primes: odd q < max_prime with q > P, or q | d (the d wheel).  The
+  // host lists those below 100000 (ascending: the tile kernel's classes);
+  // the device generates the rest (k_gen_primes, unordered).
+  std::vector<uint32_t> all, primes, dps, base;
+  odd_primes_below(std::min<uint32_t>(c.max_prime, kSmallPrimeLimit), all);
   for (uint32_t q : all) {
     if (q <= c.P) {
       if (c.d % q == 0) primes.push_back(q);
@@ -480,13 +555,16 @@ int pgx_search_create(pgx_search_t** out, pgx_runner_t* h, const uint32_t* K, ui
     }
   }
   std::vector<uint32_t>().swap(all);
-  s->n_primes = (uint32_t)primes.size();
+  s->n_small = (uint32_t)primes.size();
   s->n_tiny = (uint32_t)(std::lower_bound(primes.begin(), primes.end(), kTinyPrime) - primes.begin());
   s->n_coop = (uint32_t)(std::lower_bound(primes.begin(), primes.end(), kCoopPrime) - primes.begin());
-  s->n_small =
-      (uint32_t)(std::lower_bound(primes.begin(), primes.end(), kSmallPrimeLimit) - primes.begin());
   s->n_dp = (uint32_t)(dps.size() / 2);
-
+  size_t large_cap = 0;  // bound on the primes in [100001, max_prime): pi(x) < 1.25506 x / ln x
+  if (c.max_prime > kSmallPrimeLimit) {
+    const double x = (double)c.max_prime;
+    large_cap = (size_t)(1.25506 * x / std::log(x)) + 64;
+    odd_primes_below((uint32_t)std::sqrt(x) + 2, base);
+  }
   int rc = PGX_OK;
   auto dev = [&](uint32_t** p, size_t nwords) -> bool {
     hipError_t e = hipMalloc((void**)p, std::max<size_t>(nwords, 1) * 4);
@@ -499,8 +577,10 @@ int pgx_search_create(pgx_search_t** out, pgx_runner_t* h, const uint32_t* K, ui
   };
   hipStream_t st = h->stream;
   hipError_t e = hipSetDevice(h->device);
-  bool ok = e == hipSuccess && dev(&s->d_K, k_words) && dev(&s->d_p, s->n_primes) &&
-            dev(&s->d_nik, s->n_primes) && dev(&s->d_msm, s->n_primes) &&
+  const size_t p_cap = s->n_small + large_cap;
+  uint32_t* d_base = nullptr;
+  bool ok = e == hipSuccess && dev(&s->d_K, k_words) && dev(&s->d_p, p_cap) &&
+            dev(&s->d_nik, p_cap) && dev(&s->d_msm, p_cap) && dev(&d_base, base.size()) &&
             dev(&s->d_start, s->n_small) && dev(&s->d_dp, 2 * s->n_dp) && dev(&s->d_A, s->words) &&
             dev(&s->d_C, s->words) && dev(&s->d_F, s->words) && dev(&s->d_block, s->n_blocks) &&
             dev(&s->d_words, 4);
@@ -509,13 +589,35 @@ int pgx_search_create(pgx_search_t** out, pgx_runner_t* h, const uint32_t* K, ui
     ok = e == hipSuccess;
   }
   if (ok) e = hipMemcpy(s->d_K, K, k_words * 4, hipMemcpyHostToDevice);
-  if (ok && e == hipSuccess && s->n_primes)
+  if (ok && e == hipSuccess && s->n_small)
     e = hipMemcpy(s->d_p, primes.data(), primes.size() * 4, hipMemcpyHostToDevice);
+  if (ok && e == hipSuccess && !base.empty())
+    e = hipMemcpy(d_base, base.data(), base.size() * 4, hipMemcpyHostToDevice);
   if (ok && e == hipSuccess && s->n_dp)
     e = hipMemcpy(s->d_dp, dps.data(), dps.size() * 4, hipMemcpyHostToDevice);
   if (ok && e == hipSuccess) e = hipMemsetAsync(s->d_F, 0, s->words * 4, st);
   if (ok && e == hipSuccess) e = hipMemsetAsync(s->d_C, 0, s->words * 4, st);
   if (ok && e == hipSuccess) e = hipMemsetAsync(s->d_words, 0, 16, st);
+  s->n_primes = s->n_small;
+  if (ok && e == hipSuccess && large_cap) {  // primes in [100001, max_prime) on the device
+    const uint64_t lo = kSmallPrimeLimit + 1, hi = c.max_prime;
+    const uint64_t n_seg = (hi - lo + 2ull * kSegOdd - 1) / (2ull * kSegOdd);
+    hipLaunchKernelGGL(k_gen_primes, dim3((uint32_t)n_seg), dim3(kBlock), 0, st, d_base,
+                       (uint32_t)base.size(), lo, hi, s->d_p + s->n_small, s->d_words + 3);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(s->h_words, s->d_words, 16, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) {
+      if (s->h_words[3] > large_cap) {
+        rc = pgx::fail(h, PGX_E_HIP, "prime count %u above its bound %zu", s->h_words[3],
+                       large_cap);
+        ok = false;
+      }
+      s->n_primes = s->n_small + s->h_words[3];
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(s->d_words, 0, 16, st);
+  }
+  if (d_base) (void)hipFree(d_base);
   if (ok && e == hipSuccess && s->n_primes) {
     hipLaunchKernelGGL(k_prep, dim3((s->n_primes + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
                        s->d_K, k_words, s->d_p, s->n_primes, c.m_start, s->d_nik, s->d_msm);

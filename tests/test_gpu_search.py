@@ -109,3 +109,10 @@ def test_errors():
                                   found, (2,))
         assert nt == len(tests) > 3 * 128 and nf == len(fmi)
         assert np.array_equal(s.found()[0], np.sort(fmi))
+
+
+def test_deep_sieve_device_primes():
+    """Primes up to 2e8 generated on the device (11 M sieving primes): the
+    composites bitmap still equals the oracle's, in both arithmetic modes."""
+    run_case(337, 2310, 10**12, 20_000, 200_000_000, False, [2], 1)
+    run_case(337, 2310, 10**7, 20_000, 200_000_000, True, [2], 1)
