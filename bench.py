@@ -199,6 +199,34 @@ def bench_persistent(cands: np.ndarray, stream, peak: float, bits: int, reps: in
     }
 
 
+def bench_host_boundary(cands: np.ndarray, reps: int = 3) -> dict:
+    """The headline batches handed over at the C ABI from host memory, as the
+    drop-in does: pgx_run with candidate limbs (transpose into pinned staging,
+    H2D, kernel, D2H of the verdicts) and pgx_submit_mk with the multipliers
+    only (candidates built on the device).  PCIe and host work included."""
+    import pgx
+    K = pgx.primorial_over(P, D)
+    n = len(cands)
+    mult = np.asarray([int(v) // K for v in pgx.from_limbs(cands[:BATCH])], dtype=np.uint64)
+    x = int(pgx.from_limbs(cands[:1])[0]) - int(mult[0]) * K
+    out = {}
+    with pgx.Runner(BATCH, limbs=LIMBS) as run:
+        run.run(cands[:BATCH], pgx.MODE_REFERENCE)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for c0 in range(0, n, BATCH):
+                run.run(cands[c0:c0 + BATCH], pgx.MODE_REFERENCE)
+        out["pgx_run_tests_per_s"] = reps * n / (time.perf_counter() - t0)
+        run.run_mk(K, mult, x, 1, pgx.MODE_REFERENCE)
+        t0 = time.perf_counter()
+        for _ in range(reps * (n // BATCH)):
+            run.run_mk(K, mult, x, 1, pgx.MODE_REFERENCE)
+        out["pgx_submit_mk_tests_per_s"] = reps * (n // BATCH) * BATCH / (time.perf_counter() - t0)
+    out["workload"] = ("headline candidates in 65536-candidate batches from host memory, "
+                       "synchronous per batch (one runner, one stream)")
+    return out
+
+
 def bench_1024(stream, peak: float, reps: int) -> dict:
     """The 1024-bit class alone: P=701, d=210 (987-bit N in 32-word buffers),
     2^18 candidates; plain launches of 65536 vs one autotuned launch."""
@@ -525,7 +553,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline plain-kernel workload")
-    ap.add_argument("--extras", default="persistent,1024,mixed,large,search,serving",
+    ap.add_argument("--extras", default="persistent,host,1024,mixed,large,search,serving",
                     help="comma-separated extra configurations to run")
     ap.add_argument("--serve-group", default="1000,32768,4",
                     help="bits,items,lanes: serving iterations of classes at least `bits` wide "
@@ -634,6 +662,10 @@ def main() -> int:
         extra["same_workload_persistent"] = bench_persistent(cands, stream, peak, L,
                                                              max(2, args.steps // 4))
         log(f"persistent: {extra['same_workload_persistent']['tests_per_s'] / 1e6:.2f} M tests/s")
+    if "host" in todo:
+        extra["host_boundary"] = bench_host_boundary(cands)
+        log(f"host boundary: pgx_run {extra['host_boundary']['pgx_run_tests_per_s'] / 1e6:.2f} "
+            f"M/s, pgx_submit_mk {extra['host_boundary']['pgx_submit_mk_tests_per_s'] / 1e6:.2f} M/s")
     if "1024" in todo:
         extra["class_1024"] = bench_1024(stream, peak, max(2, args.steps // 4))
         log(f"1024-bit: plain {extra['class_1024']['plain_tests_per_s'] / 1e6:.2f} M/s, "

@@ -62,6 +62,68 @@ uint32_t bit_length(const uint32_t* x, uint32_t limbs) {
   return 0;
 }
 
+// Candidate-major rows (n x S words: the layout at the C ABI) to and from the
+// column-major layout of the kernels (word j of candidate i at j*ld + i).
+// Tiles of 64 candidates x 64 words pass through LDS so that both the reads
+// and the writes are coalesced; the grid strides over candidate tiles and
+// blockIdx.y picks the 64-word slice.
+constexpr uint32_t kTile = 64;
+
+template <bool kToCols>
+__global__ __launch_bounds__(256) void k_transpose(const uint32_t* __restrict__ src,
+                                                   uint32_t* __restrict__ dst, uint32_t n,
+                                                   uint32_t S, uint32_t ld) {
+  __shared__ uint32_t tile[kTile][kTile + 1];  // [candidate][word], odd pitch
+  const uint32_t j0 = blockIdx.y * kTile;
+  const uint32_t wj = min(kTile, S - j0);
+  const uint32_t tiles = (n + kTile - 1) / kTile;
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const uint32_t i0 = t * kTile, wi = min(kTile, n - i0);
+    for (uint32_t e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+      const uint32_t r = e / kTile, c = e % kTile;
+      if (kToCols) {  // candidate r, word c
+        if (r < wi && c < wj) tile[r][c] = src[(size_t)(i0 + r) * S + j0 + c];
+      } else {  // word r, candidate c
+        if (r < wj && c < wi) tile[c][r] = src[(size_t)(j0 + r) * ld + i0 + c];
+      }
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+      const uint32_t r = e / kTile, c = e % kTile;
+      if (kToCols) {  // word r, candidate c
+        if (r < wj && c < wi) dst[(size_t)(j0 + r) * ld + i0 + c] = tile[c][r];
+      } else {  // candidate r, word c
+        if (r < wi && c < wj) dst[(size_t)(i0 + r) * S + j0 + c] = tile[r][c];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_transpose(bool to_cols, const uint32_t* src, uint32_t* dst, size_t n,
+                            uint32_t S, uint32_t ld, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t tiles = (n + kTile - 1) / kTile;
+  const dim3 grid((unsigned)std::min<size_t>(tiles, 8192), (S + kTile - 1) / kTile);
+  if (to_cols)
+    hipLaunchKernelGGL(k_transpose<true>, grid, dim3(256), 0, s, src, dst, (uint32_t)n, S, ld);
+  else
+    hipLaunchKernelGGL(k_transpose<false>, grid, dim3(256), 0, s, src, dst, (uint32_t)n, S, ld);
+  return hipGetLastError();
+}
+
+// Queue the D2H copies of a finished batch: verdicts, and residues turned
+// candidate-major on the device (into d_cand, free once the test has run).
+int queue_results(pgx_runner* h, size_t n, uint32_t flags) {
+  PGX_HIP(h, hipMemcpyAsync(h->h_verdict, h->d_verdict, n * 4, hipMemcpyDeviceToHost, h->stream));
+  if (flags & PGX_WANT_RESIDUE) {
+    PGX_HIP(h, launch_transpose(false, h->d_res, h->d_cand, n, h->limbs, (uint32_t)n, h->stream));
+    PGX_HIP(h, hipMemcpyAsync(h->h_res, h->d_cand, (size_t)h->limbs * n * 4,
+                              hipMemcpyDeviceToHost, h->stream));
+  }
+  return PGX_OK;
+}
+
 void release(pgx_runner* h) {
   if (h->stream) (void)hipStreamDestroy(h->stream);
   if (h->h_cand) (void)hipHostFree(h->h_cand);
@@ -192,18 +254,20 @@ int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t fla
     if (L > PGX_MAX_BITS)
       return fail(h, PGX_E_INPUT, "candidate %zu has %u bits, more than the kernels support", i, L);
     max_bits = std::max(max_bits, L);
-    for (uint32_t j = 0; j < S; ++j) h->h_cand[(size_t)j * n + i] = x[j];
   }
+  // Candidate-major through pinned staging; d_res holds the rows until the
+  // transpose has read them (the test writes its residues there afterwards,
+  // in stream order).
+  std::memcpy(h->h_cand, limbs_le, (size_t)S * n * 4);
   PGX_HIP(h, hipSetDevice(h->device));
-  PGX_HIP(h, hipMemcpyAsync(h->d_cand, h->h_cand, (size_t)S * n * 4, hipMemcpyHostToDevice,
+  PGX_HIP(h, hipMemcpyAsync(h->d_res, h->h_cand, (size_t)S * n * 4, hipMemcpyHostToDevice,
                             h->stream));
+  PGX_HIP(h, launch_transpose(true, h->d_res, h->d_cand, n, S, (uint32_t)n, h->stream));
   int rc = launch_prp(h, h->d_cand, (uint32_t)n, S, n, max_bits, flags, h->d_verdict, h->d_res,
                   (uint32_t)n, h->stream);
   if (rc != PGX_OK) return rc;
-  PGX_HIP(h, hipMemcpyAsync(h->h_verdict, h->d_verdict, n * 4, hipMemcpyDeviceToHost, h->stream));
-  if (flags & PGX_WANT_RESIDUE)
-    PGX_HIP(h, hipMemcpyAsync(h->h_res, h->d_res, (size_t)S * n * 4, hipMemcpyDeviceToHost,
-                              h->stream));
+  rc = queue_results(h, n, flags);
+  if (rc != PGX_OK) return rc;
   h->pending = true;
   h->pending_n = n;
   h->pending_flags = flags;
@@ -257,10 +321,8 @@ int pgx_submit_mk(pgx_runner_t* h, const uint32_t* K, uint32_t k_words, const ui
               (uint32_t)n, h->stream);
   if (rc != PGX_OK) return rc;
   PGX_HIP(h, hipMemcpyAsync(h->h_status, h->d_status, 4, hipMemcpyDeviceToHost, h->stream));
-  PGX_HIP(h, hipMemcpyAsync(h->h_verdict, h->d_verdict, n * 4, hipMemcpyDeviceToHost, h->stream));
-  if (flags & PGX_WANT_RESIDUE)
-    PGX_HIP(h, hipMemcpyAsync(h->h_res, h->d_res, (size_t)h->limbs * n * 4, hipMemcpyDeviceToHost,
-                              h->stream));
+  rc = queue_results(h, n, flags);
+  if (rc != PGX_OK) return rc;
   h->pending = true;
   h->pending_n = n;
   h->pending_flags = flags;
@@ -280,11 +342,8 @@ int pgx_wait(pgx_runner_t* h, uint32_t* verdict, uint32_t* residues) {
     if (*h->h_status) return fail(h, PGX_E_INPUT, "a candidate m*K +- x did not fit or was negative");
   }
   if (verdict) std::memcpy(verdict, h->h_verdict, n * 4);
-  if ((h->pending_flags & PGX_WANT_RESIDUE) && residues) {
-    const uint32_t S = h->limbs;
-    for (size_t i = 0; i < n; ++i)
-      for (uint32_t j = 0; j < S; ++j) residues[i * S + j] = h->h_res[(size_t)j * n + i];
-  }
+  if ((h->pending_flags & PGX_WANT_RESIDUE) && residues)
+    std::memcpy(residues, h->h_res, (size_t)h->limbs * n * 4);
   return PGX_OK;
 }
 

@@ -19,7 +19,7 @@ struct pgx_runner {
   uint32_t limbs = 0;
   pgx::Bases bases{};
   hipStream_t stream = nullptr;
-  // pinned staging (limb-interleaved, ld = max_n)
+  // pinned staging, candidate-major as at the C ABI (the device transposes)
   uint32_t* h_cand = nullptr;
   uint32_t* h_verdict = nullptr;
   uint32_t* h_res = nullptr;

@@ -182,3 +182,21 @@ def test_device_resident_interleaved_layout():
     r = res.cpu().numpy().view(np.uint32)[:, :n].T
     ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
     assert np.array_equal(v, ov) and np.array_equal(r, orr)
+
+
+def test_host_rows_wider_than_one_tile_and_ragged_batches():
+    """pgx_run moves candidate-major rows through a device transpose in tiles of
+    64 candidates x 64 words: buffers of 67 words (a second, partial word
+    tile), batch sizes that end mid-tile, and batches smaller than the runner,
+    one after another on the same runner."""
+    rng = np.random.default_rng(11)
+    ns = [int.from_bytes(rng.bytes(250), "little") | (1 << 1990) | 1 for _ in range(200)]
+    ns += [3, 5, 2047, 2**61 - 1]
+    L = pgx.to_limbs(ns, 67)
+    ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+    with pgx.Runner(len(ns), limbs=67) as run:
+        for k in (len(ns), 1, 63, 65, 131):
+            say(f"rows: {k} candidates x 67 words")
+            v, r = run.run(L[-k:], pgx.MODE_FERMAT, residues=True)
+            assert np.array_equal(v, ov[-k:]) and np.array_equal(r, orr[-k:]), k
+            assert np.array_equal(run.run(L[:k], pgx.MODE_FERMAT), ov[:k]), k
