@@ -199,6 +199,47 @@ def bench_persistent(cands: np.ndarray, stream, peak: float, bits: int, reps: in
     }
 
 
+def bench_plain_streams(cands: np.ndarray, max_bits: int, reps: int, nstreams: int = 3) -> dict:
+    """The headline's plain launches (65,536 candidates each) spread round-robin
+    over `nstreams` streams, as the reference runs three runner instances
+    concurrently (gpu_testing.cu:203-354, one stream each): neighbouring
+    launches overlap, so a SIMD holds two wavefronts at the seams."""
+    import torch
+    import pgx
+    nb = len(cands) // BATCH
+    host = np.ascontiguousarray(cands.reshape(nb, BATCH, LIMBS).transpose(0, 2, 1))
+    dev = torch.from_numpy(host.view(np.int32)).cuda()
+    verdict = torch.zeros((nb, BATCH), dtype=torch.int32, device="cuda")
+    res = torch.zeros((nb, LIMBS, BATCH), dtype=torch.int32, device="cuda")
+    main = torch.cuda.Stream()
+    streams = [torch.cuda.Stream() for _ in range(nstreams)]
+    runner = pgx.Runner(1, limbs=LIMBS)
+
+    def step():
+        go = torch.cuda.Event()
+        go.record(main)
+        for s in streams:
+            s.wait_event(go)
+        for b in range(nb):
+            s = streams[b % nstreams]
+            runner.test_device(dev[b].data_ptr(), BATCH, LIMBS, BATCH, max_bits, pgx.MODE_FERMAT,
+                               verdict[b].data_ptr(), res[b].data_ptr(), BATCH, s.cuda_stream)
+        for s in streams:
+            done = torch.cuda.Event()
+            done.record(s)
+            main.wait_event(done)
+    step()
+    ms = time_ms(main, step, reps)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    ov = oracle.verdicts(cands[-1024:], oracle.MODE_FERMAT)
+    ok = bool(np.array_equal(verdict[-1].cpu().numpy().view(np.uint32)[-1024:], ov))
+    runner.close()
+    return {"workload": f"the headline's 16 plain launches round-robin over {nstreams} streams",
+            "ms_per_step": ms, "tests_per_s": len(cands) / (ms * 1e-3),
+            "parity_sample_ok": ok}
+
+
 def bench_host_boundary(cands: np.ndarray, reps: int = 3) -> dict:
     """The headline batches handed over at the C ABI from host memory, as the
     drop-in does: pgx_run with candidate limbs (transpose into pinned staging,
@@ -553,7 +594,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline plain-kernel workload")
-    ap.add_argument("--extras", default="persistent,host,1024,mixed,large,search,serving",
+    ap.add_argument("--extras", default="persistent,streams,host,1024,mixed,large,search,serving",
                     help="comma-separated extra configurations to run")
     ap.add_argument("--serve-group", default="1000,32768,4",
                     help="bits,items,lanes: serving iterations of classes at least `bits` wide "
@@ -662,6 +703,9 @@ def main() -> int:
         extra["same_workload_persistent"] = bench_persistent(cands, stream, peak, L,
                                                              max(2, args.steps // 4))
         log(f"persistent: {extra['same_workload_persistent']['tests_per_s'] / 1e6:.2f} M tests/s")
+    if "streams" in todo:
+        extra["plain_three_streams"] = bench_plain_streams(cands, max_bits, max(2, args.steps // 4))
+        log(f"plain on 3 streams: {extra['plain_three_streams']['tests_per_s'] / 1e6:.2f} M tests/s")
     if "host" in todo:
         extra["host_boundary"] = bench_host_boundary(cands)
         log(f"host boundary: pgx_run {extra['host_boundary']['pgx_run_tests_per_s'] / 1e6:.2f} "
