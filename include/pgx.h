@@ -55,9 +55,11 @@ int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t lim
 /* Free everything the runner owns.  Replaces miller_rabin.h:351-362. */
 int pgx_runner_destroy(pgx_runner_t* h);
 
-/* Pack n candidates (candidate-major limbs, see above) into pinned staging,
- * copy them to the device and launch asynchronously on the runner's stream.
- * Rejects even N, N <= 1 and N >= 2^(32*limbs) with PGX_E_INPUT (the reference
+/* Copy n candidates (candidate-major limbs, see above) into pinned staging,
+ * send them to the device as they are (a device kernel makes them
+ * limb-interleaved) and launch asynchronously on the runner's stream.  The
+ * caller's array may be reused as soon as this returns.  Rejects even N,
+ * N <= 1 and N wider than PGX_MAX_BITS with PGX_E_INPUT (the reference
  * asserts / exits, miller_rabin.h:46-49, 375-376). */
 int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t flags);
 

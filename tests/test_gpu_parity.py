@@ -200,3 +200,37 @@ def test_host_rows_wider_than_one_tile_and_ragged_batches():
             v, r = run.run(L[-k:], pgx.MODE_FERMAT, residues=True)
             assert np.array_equal(v, ov[-k:]) and np.array_equal(r, orr[-k:]), k
             assert np.array_equal(run.run(L[:k], pgx.MODE_FERMAT), ov[:k]), k
+
+
+def test_three_runners_driven_from_three_threads():
+    """The reference's deployment: three runner instances, each owned by its
+    own host thread and called synchronously at the same time
+    (run_gpu_thread, gpu_testing.cu:203-354). Every call returns GMP's
+    verdicts and residues for its own batch."""
+    import threading
+    m = pgx.gen_multipliers(0x5EED0007, 10**7, 10**7 + 10**4, 2310, 8)
+    L, _, _ = pgx.gen_sieved(337, 2310, m, 450000, 768, 16)
+    ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+    chunks = np.array_split(np.arange(len(L)), 9)
+    out, errors = {}, []
+
+    def worker(t):
+        try:
+            with pgx.Runner(max(len(c) for c in chunks), limbs=16) as run:
+                for rep in range(2):
+                    for k in range(t, len(chunks), 3):
+                        out[(k, rep)] = run.run(L[chunks[k]], pgx.MODE_FERMAT, residues=True)
+        except Exception as e:  # reported in the main thread
+            errors.append(e)
+
+    say(f"three threads x three runners, {len(L)} candidates, 2 passes")
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(3)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=100)
+    assert not errors, errors
+    assert len(out) == 2 * len(chunks)
+    for (k, rep), (v, r) in out.items():
+        idx = chunks[k]
+        assert np.array_equal(v, ov[idx]) and np.array_equal(r, orr[idx]), (k, rep)
