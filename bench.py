@@ -306,6 +306,45 @@ def bench_1024(stream, peak: float, reps: int) -> dict:
     }
 
 
+def bench_2048(stream, peak: float, reps: int) -> dict:
+    """The 2048-bit class: P=1447, d=210 (~2020-bit N in 64-word buffers),
+    2^16 sieved candidates, Fermat + residues: the plain one-lane launch
+    against groups of 4 lanes per candidate, and the autotuner's pick."""
+    import pgx
+    L = gen(1447, 210, 64, 16, 4096, SEED + 1447)
+    a = pgx.DeviceBatch(L)
+    run = pgx.Runner(1, limbs=64)
+
+    def plain():
+        run.test_device(a.cand.data_ptr(), a.ld, a.limbs, a.n, a.max_bits, pgx.MODE_FERMAT,
+                        a.verdict.data_ptr(), a.res.data_ptr(), a.ld, stream.cuda_stream)
+
+    def lanes4():
+        run.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True,
+                            tuning=pgx.Tuning(8, 4, 64, 4), stream=stream.cuda_stream)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    ov, ores = oracle.verdicts(L[:256], oracle.MODE_FERMAT, residues=True)
+    out = {"workload": "P=1447 d=210, 2^16 sieved candidates (~2020 bits), 64-word buffers, "
+                       "Fermat + residues", "bits": median_bits(L)}
+    w = work_per_test(out["bits"]) * a.n
+    ok = True
+    for name, fn in (("plain", plain), ("lanes4", lanes4)):
+        fn()
+        ms = time_ms(stream, fn, reps)
+        ok = ok and bool(np.array_equal(a.verdicts()[:256], ov) and
+                         np.array_equal(a.residues()[:256], ores))
+        out[f"{name}_tests_per_s"] = a.n / (ms * 1e-3)
+        out[f"{name}_valu_frac"] = w / (ms * 1e-3) / peak
+    t, rate = run.autotune(a.bucket(), pgx.MODE_FERMAT)
+    out["tuned"] = t.as_dict()
+    out["tuned_tests_per_s"] = rate
+    out["parity_sample_ok"] = ok
+    out["cpu"] = cpu_rate(L, 3.0)
+    run.close()
+    return out
+
+
 def bench_large(stream, peak: float, reps: int) -> dict:
     """P=907, d=30030 (~1260-bit N in 64-word buffers), 2^18 candidates:
     base-2 Fermat (persistent, autotuned), on-device compaction of the PRPs,
@@ -594,7 +633,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline plain-kernel workload")
-    ap.add_argument("--extras", default="persistent,streams,host,1024,mixed,large,search,serving",
+    ap.add_argument("--extras", default="persistent,streams,host,1024,2048,mixed,large,search,serving",
                     help="comma-separated extra configurations to run")
     ap.add_argument("--serve-group", default="1000,32768,4",
                     help="bits,items,lanes: serving iterations of classes at least `bits` wide "
@@ -714,6 +753,12 @@ def main() -> int:
         extra["class_1024"] = bench_1024(stream, peak, max(2, args.steps // 4))
         log(f"1024-bit: plain {extra['class_1024']['plain_tests_per_s'] / 1e6:.2f} M/s, "
             f"tuned {extra['class_1024']['tuned_tests_per_s'] / 1e6:.2f} M/s")
+    if "2048" in todo:
+        extra["class_2048"] = bench_2048(stream, peak, max(2, args.steps // 4))
+        c = extra["class_2048"]
+        log(f"2048-bit: plain {c['plain_tests_per_s'] / 1e6:.3f} M/s, 4 lanes "
+            f"{c['lanes4_tests_per_s'] / 1e6:.3f} M/s, tuned {c['tuned_tests_per_s'] / 1e6:.3f} M/s "
+            f"{c['tuned']}, cpu {c['cpu']['tests_per_s']:.0f}/s")
     if "mixed" in todo:
         extra["mixed_graph"] = bench_mixed(stream, peak, max(2, args.steps // 4))
     if "large" in todo:
