@@ -296,15 +296,15 @@ __device__ __forceinline__ void mul_small(uint32_t (&x)[Q], uint32_t a, const ui
 }
 
 // Exponentiation of x = Mont(base) by N-1 with the strong-test checks (the
-// group form of powm_verdict in prp.hpp).  res <- base^(N-1) mod N.
+// group form of powm_verdict in prp.hpp); one = R mod N from the caller.
+// res <- base^(N-1) mod N.
 template <int Q, int G, class WordFn>
-__device__ __forceinline__ uint32_t powm_verdict(uint32_t (&x)[Q], const uint32_t (&n)[Q],
-                                                 uint32_t ninv, int L, int t, uint32_t base,
-                                                 double inv_n, WordFn word, uint32_t (&res)[Q]) {
-  uint32_t one[Q];
+__device__ __forceinline__ uint32_t powm_verdict(uint32_t (&x)[Q], const uint32_t (&one)[Q],
+                                                 const uint32_t (&n)[Q], uint32_t ninv, int L,
+                                                 int t, uint32_t base, double inv_n, WordFn word,
+                                                 uint32_t (&res)[Q]) {
   bool sprp = false;
   if (t == L - 1) {
-    r_mod_n<Q, G>(one, n, L);
     canon<Q, G>(x, n);
     sprp = equal<Q, G>(x, one) || is_neg<Q, G>(x, one, n);
   }
@@ -316,7 +316,6 @@ __device__ __forceinline__ uint32_t powm_verdict(uint32_t (&x)[Q], const uint32_
     if (base == 2u) dbl_lazy<Q, G>(x, bit);
     else if (bit) mul_small<Q, G>(x, base, n, inv_n, L);
     if (p <= t && p >= 1) {
-      if (p == t) r_mod_n<Q, G>(one, n, L);
       canon<Q, G>(x, n);
       sprp = sprp || is_neg<Q, G>(x, one, n) || (p == t && equal<Q, G>(x, one));
     }

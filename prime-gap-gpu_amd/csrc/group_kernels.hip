@@ -24,7 +24,7 @@ __device__ __forceinline__ uint32_t group_test_base(uint32_t base, const uint32_
   grp::sub_if_ge<Q, G>(x, n);
   const size_t ld = a.ld;
   auto word = [&](int j) -> uint32_t { return (j < (int)a.buf_limbs) ? c[(size_t)j * ld] : 0u; };
-  return grp::powm_verdict<Q, G>(x, n, ninv, L, t, base, inv_n, word, res);
+  return grp::powm_verdict<Q, G>(x, one, n, ninv, L, t, base, inv_n, word, res);
 }
 
 template <int Q, int G>

@@ -64,16 +64,18 @@ struct MulBySmall {
 };
 
 // Exponentiation + strong-test tail.  x enters as Mont(a) (the leading bit of
-// N-1).  `word(j)` returns 32-bit word j of N; it is read once per 32 exponent
-// bits so the exponent never needs dynamically indexed registers.  res gets
-// a^(N-1) mod N (canonical); returns FLAG_* bits.
+// N-1); one = Mont(1) = R mod N, computed once by the caller (the tail
+// compares against it: recomputing it where p == t would repeat it for every
+// distinct t among a wavefront's lanes).  `word(j)` returns 32-bit word j of
+// N; it is read once per 32 exponent bits so the exponent never needs
+// dynamically indexed registers.  res gets a^(N-1) mod N (canonical); returns
+// FLAG_* bits.
 template <int S, class WordFn, class Step>
-PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv, int L, int t,
-                             WordFn word, const Step& step, uint32_t (&res)[S]) {
-  uint32_t one[S];
+PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&one)[S], const uint32_t (&n)[S],
+                             uint32_t ninv, int L, int t, WordFn word, const Step& step,
+                             uint32_t (&res)[S]) {
   bool sprp = false;
   if (t == L - 1) {  // N - 1 = 2^t: y_0 = a
-    r_mod_n<S>(one, n, L);
     canon<S>(x, n);
     sprp = equal<S>(x, one) || is_neg<S>(x, one, n);
   }
@@ -89,7 +91,6 @@ PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t 
       step(x, n, ninv, bit);
     }
     if (p <= t && p >= 1) {  // x = Mont(y_{t-p})
-      if (p == t) r_mod_n<S>(one, n, L);
       canon<S>(x, n);
       sprp = sprp || is_neg<S>(x, one, n) || (p == t && equal<S>(x, one));
     }
@@ -113,33 +114,29 @@ PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t 
 template <int S, bool kPreShift, class WordFn>
 PGX_FN uint32_t test_base2(const uint32_t (&n)[S], uint32_t ninv, int L, int t, WordFn word,
                            uint32_t (&res)[S]) {
-  uint32_t x[S];
-  r_mod_n<S>(x, n, L);
+  uint32_t one[S], x[S];
+  r_mod_n<S>(one, n, L);
+#pragma unroll
+  for (int j = 0; j < S; ++j) x[j] = one[j];
   dbl_full<S>(x, n);  // Mont(2)
-  return powm_verdict<S>(x, n, ninv, L, t, word, MulBy2<S, kPreShift>{}, res);
+  return powm_verdict<S>(x, one, n, ninv, L, t, word, MulBy2<S, kPreShift>{}, res);
 }
 
 // Base 2 <= a < 2^8 (the extra Miller-Rabin bases).
 template <int S, class WordFn>
 PGX_FN uint32_t test_base(uint32_t a, const uint32_t (&n)[S], uint32_t ninv, int L, int t,
                           WordFn word, uint32_t (&res)[S]) {
-  uint32_t am[S];
-  {
-    uint32_t one[S];
-    r_mod_n<S>(one, n, L);
-    // am = a * R mod N by double-and-add over the bits of a
+  uint32_t one[S], x[S];
+  r_mod_n<S>(one, n, L);
+  // x = a * R mod N by double-and-add over the bits of a
 #pragma unroll
-    for (int j = 0; j < S; ++j) am[j] = 0;
-    for (int b = 31 - __builtin_clz(a); b >= 0; --b) {
-      dbl_full<S>(am, n);
-      if ((a >> b) & 1u) add_full<S>(am, one, n);
-    }
+  for (int j = 0; j < S; ++j) x[j] = 0;
+  for (int b = 31 - __builtin_clz(a); b >= 0; --b) {
+    dbl_full<S>(x, n);
+    if ((a >> b) & 1u) add_full<S>(x, one, n);
   }
-  uint32_t x[S];
-#pragma unroll
-  for (int j = 0; j < S; ++j) x[j] = am[j];
   const double inv_n = 1.0 / scaled<S>(n, L);
-  return powm_verdict<S>(x, n, ninv, L, t, word, MulBySmall<S>{a, inv_n, L}, res);
+  return powm_verdict<S>(x, one, n, ninv, L, t, word, MulBySmall<S>{a, inv_n, L}, res);
 }
 
 // Whole test of one candidate given as W32 words (w).  Returns the verdict word
