@@ -500,7 +500,7 @@ def bench_search(n_x: int = 8) -> dict:
 
 def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: int = 100,
                   group_min_bits: int = 0, group_max_items: int = 0,
-                  group_lanes: int = 4) -> dict:
+                  group_lanes: int = 4, wide_cu_percent: int = 0) -> dict:
     """Poisson stream of per-m next/prev-prime requests (P in {337, 503, 907},
     d=2310, 20-500 sieved X per side), continuous batching under a 64 GB cap.
     Each request is submitted at its scheduled arrival time; latency runs from
@@ -530,7 +530,8 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
     pick = rng.integers(0, len(tmpl), size=n_req)
     results, lat, lat_by_p = {}, [], {}
     with pgx.Server(hbm_budget=64 << 30, chunk=32, group_min_bits=group_min_bits,
-                    group_max_items=group_max_items, group_lanes=group_lanes) as srv:
+                    group_max_items=group_max_items, group_lanes=group_lanes,
+                    wide_cu_percent=wide_cu_percent) as srv:
         for k in range(len(sizes)):  # warm each width class (allocation, code load)
             P, bits, nxt, prv = tmpl[k * templates_per_p]
             srv.submit(10**9 + k, nxt, prv, bits)
@@ -635,10 +636,11 @@ def main() -> int:
                     help="only the headline plain-kernel workload")
     ap.add_argument("--extras", default="persistent,streams,host,1024,2048,mixed,large,search,serving",
                     help="comma-separated extra configurations to run")
-    ap.add_argument("--serve-group", default="1000,32768,4",
-                    help="bits,items,lanes: serving iterations of classes at least `bits` wide "
-                         "with at most `items` candidates use `lanes` lanes per candidate "
-                         "(0,0,4 = never)")
+    ap.add_argument("--serve-group", default="1000,32768,4,88",
+                    help="bits,items,lanes[,cu%%]: serving iterations of classes at least `bits` "
+                         "wide with at most `items` candidates use `lanes` lanes per candidate "
+                         "(0,0,4 = never); cu%% > 0 runs those classes on that share of the CUs "
+                         "and the narrower classes on the rest")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -768,8 +770,12 @@ def main() -> int:
         log(f"search: sieve {extra['search_interval']['sieve_ms_per_x']:.2f} ms/X, "
             f"tests {extra['search_interval']['test_ms_per_x']:.2f} ms/X")
     if "serving" in todo:
-        gb, gi, gl = (int(v) for v in args.serve_group.split(","))
-        srv = bench_serving(group_min_bits=gb, group_max_items=gi, group_lanes=gl)
+        sg = [int(v) for v in args.serve_group.split(",")]
+        gb, gi, gl = sg[:3]
+        wide = sg[3] if len(sg) > 3 else 0
+        srv = bench_serving(group_min_bits=gb, group_max_items=gi, group_lanes=gl,
+                            wide_cu_percent=wide)
+        srv["wide_cu_percent"] = wide
         # the same stream with one lane per candidate in every class
         one = bench_serving(group_min_bits=0, group_max_items=0, group_lanes=4)
         srv["one_lane_per_candidate"] = {

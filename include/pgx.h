@@ -288,6 +288,10 @@ typedef struct {
   uint32_t group_max_items; /* ... iterations of at most this many candidates with ... */
   uint32_t group_lanes;     /* ... 2 or 4 lanes per candidate (lower latency for the wide
                                classes on a lightly loaded GPU); group_max_items 0 = never */
+  uint32_t wide_cu_percent; /* 1..99: the classes at least group_min_bits wide run on this
+                               share of the CUs and the narrower ones on the rest, so a
+                               narrow class's wavefronts never share a SIMD with a wide
+                               class's; 0 = every class may use every CU */
 } pgx_serve_config_t;
 
 typedef struct {

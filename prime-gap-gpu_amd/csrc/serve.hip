@@ -177,9 +177,20 @@ int make_class(pgx_server* s, int S, Class** out) {
   if (e == hipSuccess) {
     // A stream created with a CU mask gets a hardware queue of its own, so a
     // narrow class never waits behind a wide class's kernels in a queue that
-    // the runtime shares between streams.  The mask holds every CU.
-    std::vector<uint32_t> mask((s->h->n_cu + 31) / 32, 0u);
-    for (int cu = 0; cu < s->h->n_cu; ++cu) mask[cu / 32] |= 1u << (cu % 32);
+    // the runtime shares between streams.  The mask holds every CU, or with
+    // wide_cu_percent the wide classes' share [0, n_wide) or the narrow
+    // classes' [n_wide, n_cu): a wide class's wavefronts then never take the
+    // SIMDs that a narrow class's short iterations run on.
+    const int n_cu = s->h->n_cu;
+    int lo = 0, hi = n_cu;
+    if (s->cfg.wide_cu_percent) {
+      const int n_wide = std::min(n_cu - 1, std::max(1, n_cu * (int)s->cfg.wide_cu_percent / 100));
+      const bool wide = s->cfg.group_min_bits && bits >= s->cfg.group_min_bits;
+      lo = wide ? 0 : n_wide;
+      hi = wide ? n_wide : n_cu;
+    }
+    std::vector<uint32_t> mask((n_cu + 31) / 32, 0u);
+    for (int cu = lo; cu < hi; ++cu) mask[cu / 32] |= 1u << (cu % 32);
     e = hipExtStreamCreateWithCUMask(&c->stream, (uint32_t)mask.size(), mask.data());
   }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->done, hipEventDisableTiming);
@@ -337,7 +348,8 @@ int pgx_serve_create(pgx_server_t** out, int device, const pgx_serve_config_t* c
   *out = nullptr;
   if (cfg->chunk == 0 || cfg->max_requests == 0 || cfg->pool_columns == 0 ||
       cfg->max_side == 0 || cfg->max_side * 2ull > cfg->pool_columns ||
-      (cfg->group_max_items && cfg->group_lanes != 2 && cfg->group_lanes != 4))
+      (cfg->group_max_items && cfg->group_lanes != 2 && cfg->group_lanes != 4) ||
+      cfg->wide_cu_percent > 99 || (cfg->wide_cu_percent && !cfg->group_min_bits))
     return fail(nullptr, PGX_E_ARG, "bad serving config");
   const uint32_t base2 = 2;
   pgx_runner_t* h = nullptr;

@@ -60,11 +60,14 @@ def drain(srv, want, timeout=60.0):
     return got
 
 
-@pytest.mark.parametrize("group", [(0, 0, 4), (400, 1 << 20, 4), (400, 1 << 20, 2)])
+@pytest.mark.parametrize("group", [(0, 0, 4, 0), (400, 1 << 20, 4, 0), (400, 1 << 20, 2, 0),
+                                   (600, 1 << 20, 4, 75)])
 def test_alone_and_mixed_equal_oracle(requests, expected, group):
-    """Also with the wide classes' iterations on 2- or 4-lane groups."""
-    gb, gi, gl = group
-    with pgx.Server(chunk=16, group_min_bits=gb, group_max_items=gi, group_lanes=gl) as srv:
+    """Also with the wide classes' iterations on 2- or 4-lane groups, and with
+    the CUs split between wide and narrow classes."""
+    gb, gi, gl, wide = group
+    with pgx.Server(chunk=16, group_min_bits=gb, group_max_items=gi, group_lanes=gl,
+                    wide_cu_percent=wide) as srv:
         alone = {}
         for i, (P, words, nxt, prv) in enumerate(requests):
             srv.submit(i, nxt, prv, bits(P))
@@ -103,6 +106,10 @@ def test_budget_forces_queueing(requests, expected):
 def test_rejections_and_empty_request(requests):
     with pytest.raises(pgx.PgxError):
         pgx.Server(group_max_items=64, group_lanes=3)
+    with pytest.raises(pgx.PgxError):  # a CU split needs the wide/narrow boundary
+        pgx.Server(wide_cu_percent=50)
+    with pytest.raises(pgx.PgxError):
+        pgx.Server(group_min_bits=1000, wide_cu_percent=100)
     P, words, nxt, prv = requests[0]
     with pgx.Server(hbm_budget=1000, max_side=512) as srv:
         with pytest.raises(pgx.PgxError) as e:  # needs more than the budget
