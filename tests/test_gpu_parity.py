@@ -234,3 +234,25 @@ def test_three_runners_driven_from_three_threads():
     for (k, rep), (v, r) in out.items():
         idx = chunks[k]
         assert np.array_equal(v, ov[idx]) and np.array_equal(r, orr[idx]), (k, rep)
+
+
+@pytest.mark.parametrize("S", [5, 17, 31, 46, 74])
+def test_bit_lengths_below_kernel_top(S):
+    """Random odd N of every bit length from 60 bits below the widest the
+    kernel of S limbs holds up to it, in one batch (the kernel is picked by the
+    widest, so R/N runs from 2^4 to 2^64 across the batch); residues and
+    verdicts against GMP."""
+    import random
+    rb = 29 if S <= 31 else 28
+    top = rb * S - 4
+    rng = random.Random(S)
+    vals = [rng.getrandbits(b) | (1 << (b - 1)) | 1
+            for b in range(top - 60, top + 1) for _ in range(8)]
+    words = (top + 31) // 32
+    L = pgx.to_limbs(vals, words)
+    say(f"S={S}: {len(vals)} candidates of {top - 60}..{top} bits")
+    ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+    with pgx.Runner(len(vals), limbs=words, bases=[2, 3]) as run:
+        v, r = run.run(L, pgx.MODE_FERMAT, residues=True)
+        assert np.array_equal(v, ov) and np.array_equal(r, orr)
+        assert np.array_equal(run.run(L, pgx.MODE_STRONG), O.verdicts(L, O.MODE_STRONG, (2, 3)))

@@ -158,6 +158,24 @@ def test_lane_routine_sieved_vs_oracle(emu):
     assert np.array_equal(v, O.verdicts(L, O.MODE_REFERENCE, (2, 3, 5)))
 
 
+def test_lane_routine_every_width_gap(emu):
+    """Random odd N of every bit length from 60 bits below a kernel's top up
+    to its top (R/N from 2^64 down to 2^4: Mont(1) = R mod N after 5 to 65
+    doublings), residues and verdicts against GMP."""
+    import random
+    rng = random.Random(5)
+    for S in (5, 12, 17, 26):
+        top = 29 * S - 4
+        vals = [rng.getrandbits(b) | (1 << (b - 1)) | 1
+                for b in range(top - 60, top + 1) for _ in range(4)]
+        L = pgx.to_limbs(vals, (29 * S + 31) // 32)
+        v, r = emu(L, S, pgx.MODE_FERMAT)
+        ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+        assert np.array_equal(v, ov) and np.array_equal(r, orr), S
+        v, _ = emu(L, S, pgx.MODE_STRONG, (2, 3, 5))
+        assert np.array_equal(v, O.verdicts(L, O.MODE_STRONG, (2, 3, 5))), S
+
+
 def test_lane_routine_rejects_what_does_not_fit(emu):
     # even, one, and a value too wide for one radix-2^29 limb (R = 2^29 >= 16N
     # allows 25-bit N)
