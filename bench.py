@@ -826,6 +826,10 @@ def main() -> int:
             "macs_per_launch": macs_launch,
             "avg_launch_ms": avg_launch_ms,
             "valu_rates_per_s": rate,
+            # the same launches against HBM (8 TB/s published): far from that bound
+            "hbm": {"achieved_gbs": bytes_launch / (avg_launch_ms * 1e-3) / 1e9,
+                    "peak_gbs": 8000.0,
+                    "frac": bytes_launch / (avg_launch_ms * 1e-3) / 8e12},
         },
         "cpu_baseline": cpu,
         "parity_sample_ok": parity,
