@@ -500,7 +500,8 @@ def bench_search(n_x: int = 8) -> dict:
 
 def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: int = 100,
                   group_min_bits: int = 0, group_max_items: int = 0,
-                  group_lanes: int = 4, wide_cu_percent: int = 0) -> dict:
+                  group_lanes: int = 4, wide_cu_percent: int = 0,
+                  slots_per_class: int = 1) -> dict:
     """Poisson stream of per-m next/prev-prime requests (P in {337, 503, 907},
     d=2310, 20-500 sieved X per side), continuous batching under a 64 GB cap.
     Each request is submitted at its scheduled arrival time; latency runs from
@@ -531,7 +532,7 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
     results, lat, lat_by_p = {}, [], {}
     with pgx.Server(hbm_budget=64 << 30, chunk=32, group_min_bits=group_min_bits,
                     group_max_items=group_max_items, group_lanes=group_lanes,
-                    wide_cu_percent=wide_cu_percent) as srv:
+                    wide_cu_percent=wide_cu_percent, slots_per_class=slots_per_class) as srv:
         for k in range(len(sizes)):  # warm each width class (allocation, code load)
             P, bits, nxt, prv = tmpl[k * templates_per_p]
             srv.submit(10**9 + k, nxt, prv, bits)
@@ -637,10 +638,11 @@ def main() -> int:
     ap.add_argument("--extras", default="persistent,streams,host,1024,2048,mixed,large,search,serving",
                     help="comma-separated extra configurations to run")
     ap.add_argument("--serve-group", default="1000,32768,4,88",
-                    help="bits,items,lanes[,cu%%]: serving iterations of classes at least `bits` "
-                         "wide with at most `items` candidates use `lanes` lanes per candidate "
-                         "(0,0,4 = never); cu%% > 0 runs those classes on that share of the CUs "
-                         "and the narrower classes on the rest")
+                    help="bits,items,lanes[,cu%%[,slots]]: serving iterations of classes at "
+                         "least `bits` wide with at most `items` candidates use `lanes` lanes per "
+                         "candidate (0,0,4 = never); cu%% > 0 runs those classes on that share of "
+                         "the CUs and the narrower classes on the rest; `slots` iterations of a "
+                         "class may be in flight at once")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -773,9 +775,11 @@ def main() -> int:
         sg = [int(v) for v in args.serve_group.split(",")]
         gb, gi, gl = sg[:3]
         wide = sg[3] if len(sg) > 3 else 0
+        slots = sg[4] if len(sg) > 4 else 1
         srv = bench_serving(group_min_bits=gb, group_max_items=gi, group_lanes=gl,
-                            wide_cu_percent=wide)
+                            wide_cu_percent=wide, slots_per_class=slots)
         srv["wide_cu_percent"] = wide
+        srv["slots_per_class"] = slots
         # the same stream with one lane per candidate in every class
         one = bench_serving(group_min_bits=0, group_max_items=0, group_lanes=4)
         srv["one_lane_per_candidate"] = {

@@ -292,6 +292,9 @@ typedef struct {
                                share of the CUs and the narrower ones on the rest, so a
                                narrow class's wavefronts never share a SIMD with a wide
                                class's; 0 = every class may use every CU */
+  uint32_t slots_per_class; /* iterations of one class in flight at once (1..4; 0 = 1): with
+                               2, requests arriving during an iteration start at once in
+                               the other slot instead of waiting for it to finish */
 } pgx_serve_config_t;
 
 typedef struct {

@@ -13,15 +13,19 @@
 // allocator; admission needs a free column range and room under the byte
 // budget (in-use candidate and verdict bytes of admitted requests).
 //
-// Iteration-level scheduling: each class runs iterations on its own stream.
-// An iteration takes the next `chunk` untested candidates of every unfinished
-// side of every admitted request of the class, tests them with one indexed
-// launch, and gathers their verdicts to the host.  pgx_serve_poll advances
-// every class whose iteration has finished: it retires sides that found a PRP
-// (early exit) or ran out of candidates, completes requests, frees their
-// memory, admits waiting requests into the freed room and launches the class's
-// next iteration.  Requests that arrive while others are on the GPU join the
-// next iteration of their class, and classes overlap on the device.
+// Iteration-level scheduling: each class runs iterations in up to
+// `slots_per_class` slots, each slot with its own stream.  An iteration takes
+// the next `chunk` untested candidates of every unfinished side that no other
+// iteration holds, of every admitted request of the class, tests them with
+// one indexed launch, and gathers their verdicts to the host.  pgx_serve_poll
+// applies every finished iteration: it retires sides that found a PRP (early
+// exit) or ran out of candidates, completes requests, frees their memory,
+// admits waiting requests into the freed room and launches an iteration in a
+// free slot.  Requests that arrive while an iteration runs join the next one,
+// which with two slots starts at once beside it; classes overlap on the
+// device.  A side is in at most one iteration at a time, so a request's
+// admission copy (on the stream of the slot whose iteration first reads it)
+// is ordered before every read of its columns.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -91,6 +95,7 @@ struct Request {
   uint32_t cursor[2] = {0, 0};  // candidates tested so far per side
   int32_t first[2] = {-1, -1};  // index of the first PRP per side
   bool done[2] = {false, false};
+  bool busy[2] = {false, false};  // a chunk of the side is in an unfinished iteration
   uint32_t col0 = 0;            // pool columns [col0, col0 + n_next + n_prev)
   uint32_t tests = 0;
   std::vector<uint32_t> cand;   // host copy until admitted: W rows x (n_next + n_prev)
@@ -103,6 +108,19 @@ struct Segment {
   uint32_t off, count;
 };
 
+// One iteration in flight at a time: its stream, buffers and segments.
+struct Slot {
+  uint32_t* d_index = nullptr;
+  uint32_t* d_out = nullptr;
+  uint32_t* h_index = nullptr;  // pinned
+  uint32_t* h_out = nullptr;    // pinned
+  uint32_t* h_stage = nullptr;  // pinned admission staging, W rows x cap_stage
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  bool in_flight = false;
+  std::vector<Segment> segs;
+};
+
 struct Class {
   int S = 0;
   uint32_t W = 0;      // 32-bit words per candidate stored in the pool
@@ -111,17 +129,9 @@ struct Class {
   std::unique_ptr<Columns> cols;
   uint32_t* d_pool = nullptr;
   uint32_t* d_verdict = nullptr;
-  uint32_t* d_index = nullptr;
-  uint32_t* d_out = nullptr;
-  uint32_t* h_index = nullptr;  // pinned
-  uint32_t* h_out = nullptr;    // pinned
-  uint32_t* h_stage = nullptr;  // pinned admission staging, W rows x cap_stage
   uint32_t cap_stage = 0;
   uint32_t max_items = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;
-  bool in_flight = false;
-  std::vector<Segment> segs;
+  std::vector<Slot> slots;
   std::vector<std::unique_ptr<Request>> active;
   std::deque<std::unique_ptr<Request>> waiting;
 };
@@ -140,12 +150,16 @@ struct pgx_server {
 namespace {
 
 void release_class(Class& c) {
-  if (c.done) (void)hipEventDestroy(c.done);
-  if (c.stream) (void)hipStreamDestroy(c.stream);
-  for (uint32_t* p : {c.d_pool, c.d_verdict, c.d_index, c.d_out})
+  for (Slot& q : c.slots) {
+    if (q.done) (void)hipEventDestroy(q.done);
+    if (q.stream) (void)hipStreamDestroy(q.stream);
+    for (uint32_t* p : {q.d_index, q.d_out})
+      if (p) (void)hipFree(p);
+    for (uint32_t* p : {q.h_index, q.h_out, q.h_stage})
+      if (p) (void)hipHostFree(p);
+  }
+  for (uint32_t* p : {c.d_pool, c.d_verdict})
     if (p) (void)hipFree(p);
-  for (uint32_t* p : {c.h_index, c.h_out, c.h_stage})
-    if (p) (void)hipHostFree(p);
 }
 
 int make_class(pgx_server* s, int S, Class** out) {
@@ -166,14 +180,17 @@ int make_class(pgx_server* s, int S, Class** out) {
   c->cols = std::make_unique<Columns>(c->cap);
   c->max_items = s->cfg.max_requests * 2u * c->chunk;
   c->cap_stage = std::max(s->cfg.stage_columns, s->cfg.max_side * 2u);
+  c->slots.resize(s->cfg.slots_per_class ? s->cfg.slots_per_class : 1);
   hipError_t e = hipMalloc((void**)&c->d_pool, (size_t)c->W * c->cap * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&c->d_verdict, (size_t)c->cap * 4);
-  if (e == hipSuccess) e = hipMalloc((void**)&c->d_index, (size_t)c->max_items * 4);
-  if (e == hipSuccess) e = hipMalloc((void**)&c->d_out, (size_t)c->max_items * 4);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_index, (size_t)c->max_items * 4, 0);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_out, (size_t)c->max_items * 4, 0);
-  if (e == hipSuccess)
-    e = hipHostMalloc((void**)&c->h_stage, (size_t)c->W * c->cap_stage * 4, 0);
+  for (Slot& q : c->slots) {
+    if (e == hipSuccess) e = hipMalloc((void**)&q.d_index, (size_t)c->max_items * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&q.d_out, (size_t)c->max_items * 4);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&q.h_index, (size_t)c->max_items * 4, 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&q.h_out, (size_t)c->max_items * 4, 0);
+    if (e == hipSuccess)
+      e = hipHostMalloc((void**)&q.h_stage, (size_t)c->W * c->cap_stage * 4, 0);
+  }
   if (e == hipSuccess) {
     // A stream created with a CU mask gets a hardware queue of its own, so a
     // narrow class never waits behind a wide class's kernels in a queue that
@@ -191,9 +208,12 @@ int make_class(pgx_server* s, int S, Class** out) {
     }
     std::vector<uint32_t> mask((n_cu + 31) / 32, 0u);
     for (int cu = lo; cu < hi; ++cu) mask[cu / 32] |= 1u << (cu % 32);
-    e = hipExtStreamCreateWithCUMask(&c->stream, (uint32_t)mask.size(), mask.data());
+    for (Slot& q : c->slots) {
+      if (e == hipSuccess)
+        e = hipExtStreamCreateWithCUMask(&q.stream, (uint32_t)mask.size(), mask.data());
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&q.done, hipEventDisableTiming);
+    }
   }
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->done, hipEventDisableTiming);
   if (e != hipSuccess) {
     release_class(*c);
     return fail(s->h, PGX_E_HIP, "serving class allocation: %s", hipGetErrorString(e));
@@ -204,8 +224,9 @@ int make_class(pgx_server* s, int S, Class** out) {
 }
 
 // Moves waiting requests of the class into the pool while room allows
-// (FIFO: a request that does not fit blocks the ones behind it).
-int admit(pgx_server* s, Class& c, uint32_t* staged_out) {
+// (FIFO: a request that does not fit blocks the ones behind it), through the
+// staging and on the stream of slot q, whose next iteration reads them first.
+int admit(pgx_server* s, Class& c, Slot& q, uint32_t* staged_out) {
   uint32_t staged = 0;
   while (!c.waiting.empty() && c.active.size() < s->cfg.max_requests) {
     Request& r = *c.waiting.front();
@@ -216,10 +237,10 @@ int admit(pgx_server* s, Class& c, uint32_t* staged_out) {
     if (at == UINT32_MAX) break;
     r.col0 = at;
     // pack into this iteration's staging slice, then one 2D copy per request
-    uint32_t* st = c.h_stage + (size_t)staged * c.W;
+    uint32_t* st = q.h_stage + (size_t)staged * c.W;
     std::memcpy(st, r.cand.data(), (size_t)c.W * n * 4);
     PGX_HIP(s->h, hipMemcpy2DAsync(c.d_pool + at, (size_t)c.cap * 4, st, (size_t)n * 4,
-                                   (size_t)n * 4, c.W, hipMemcpyHostToDevice, c.stream));
+                                   (size_t)n * 4, c.W, hipMemcpyHostToDevice, q.stream));
     staged += n;
     s->in_use += r.bytes;
     r.cand.clear();
@@ -231,32 +252,34 @@ int admit(pgx_server* s, Class& c, uint32_t* staged_out) {
   return PGX_OK;
 }
 
-// Builds and launches the next iteration of the class.  The class event is
-// recorded whenever anything was enqueued (admission copies included), so the
+// Builds and launches an iteration of the class in slot q over every side
+// that is neither decided nor held by another iteration.  The slot's event is
+// recorded whenever anything was enqueued (admission copies included), so its
 // pinned staging is never refilled while a copy from it may be pending.
-int launch_iteration(pgx_server* s, Class& c, bool copies_pending) {
-  c.segs.clear();
+int launch_iteration(pgx_server* s, Class& c, Slot& q, bool copies_pending) {
+  q.segs.clear();
   uint32_t items = 0;
   for (auto& rp : c.active) {
     Request& r = *rp;
     for (int side = 0; side < 2; ++side) {
-      if (r.done[side]) continue;
+      if (r.done[side] || r.busy[side]) continue;
       const uint32_t k = std::min(c.chunk, r.n_side[side] - r.cursor[side]);
       const uint32_t base = r.col0 + (side ? r.n_side[0] : 0) + r.cursor[side];
-      for (uint32_t j = 0; j < k; ++j) c.h_index[items + j] = base + j;
-      c.segs.push_back(Segment{&r, side, items, k});
+      for (uint32_t j = 0; j < k; ++j) q.h_index[items + j] = base + j;
+      q.segs.push_back(Segment{&r, side, items, k});
+      r.busy[side] = true;
       items += k;
     }
   }
   if (items == 0) {
     if (copies_pending) {
-      PGX_HIP(s->h, hipEventRecord(c.done, c.stream));
-      c.in_flight = true;
+      PGX_HIP(s->h, hipEventRecord(q.done, q.stream));
+      q.in_flight = true;
     }
     return PGX_OK;
   }
-  PGX_HIP(s->h, hipMemcpyAsync(c.d_index, c.h_index, (size_t)items * 4, hipMemcpyHostToDevice,
-                               c.stream));
+  PGX_HIP(s->h, hipMemcpyAsync(q.d_index, q.h_index, (size_t)items * 4, hipMemcpyHostToDevice,
+                               q.stream));
   // a small iteration of a wide class: a group of lanes per candidate
   const uint32_t cap_bits = (uint32_t)pgx::kernel_capacity_bits(c.S);
   const int G = (int)s->cfg.group_lanes;
@@ -272,32 +295,34 @@ int launch_iteration(pgx_server* s, Class& c, bool copies_pending) {
   a.mode = PGX_MODE_REFERENCE;
   a.bases = s->h->bases;  // {2}
   a.verdict = c.d_verdict;
-  a.index = c.d_index;
-  PGX_HIP(s->h, (Sp ? pgx::group_table(G)[Sp] : pgx::prp_table()[c.S])(a, c.stream));
-  PGX_HIP(s->h, pgx::launch_gather(c.d_verdict, c.d_index, items, c.d_out, c.stream));
-  PGX_HIP(s->h, hipMemcpyAsync(c.h_out, c.d_out, (size_t)items * 4, hipMemcpyDeviceToHost,
-                               c.stream));
-  PGX_HIP(s->h, hipEventRecord(c.done, c.stream));
-  c.in_flight = true;
+  a.index = q.d_index;
+  PGX_HIP(s->h, (Sp ? pgx::group_table(G)[Sp] : pgx::prp_table()[c.S])(a, q.stream));
+  PGX_HIP(s->h, pgx::launch_gather(c.d_verdict, q.d_index, items, q.d_out, q.stream));
+  PGX_HIP(s->h, hipMemcpyAsync(q.h_out, q.d_out, (size_t)items * 4, hipMemcpyDeviceToHost,
+                               q.stream));
+  PGX_HIP(s->h, hipEventRecord(q.done, q.stream));
+  q.in_flight = true;
   return PGX_OK;
 }
 
-// Applies a finished iteration's verdicts (segments of requests still active).
-void apply(pgx_server* s, Class& c) {
-  for (const Segment& g : c.segs) {
+// Applies a finished iteration's verdicts (segments of requests still active:
+// a request leaves only when neither side is held by an iteration).
+void apply(pgx_server* s, Slot& q) {
+  for (const Segment& g : q.segs) {
     Request& r = *g.req;
     r.tests += g.count;
     s->tests_total += g.count;
     for (uint32_t j = 0; j < g.count; ++j)
-      if (c.h_out[g.off + j] == 1u) {  // the single base (2) passed
+      if (q.h_out[g.off + j] == 1u) {  // the single base (2) passed
         r.first[g.side] = (int32_t)(r.cursor[g.side] + j);
         r.done[g.side] = true;
         break;
       }
     r.cursor[g.side] += g.count;
     if (r.cursor[g.side] >= r.n_side[g.side]) r.done[g.side] = true;
+    r.busy[g.side] = false;
   }
-  c.segs.clear();
+  q.segs.clear();
 }
 
 // Requests with both sides decided leave the class and free their memory.
@@ -324,19 +349,24 @@ void complete(pgx_server* s, Class& c) {
 }
 
 int advance(pgx_server* s, Class& c) {
-  if (c.in_flight) {
-    hipError_t q = hipEventQuery(c.done);
-    if (q == hipErrorNotReady) return PGX_OK;
-    if (q != hipSuccess)
-      return fail(s->h, PGX_E_HIP, "serving iteration: %s", hipGetErrorString(q));
-    c.in_flight = false;
-    apply(s, c);
+  Slot* free_slot = nullptr;
+  for (Slot& q : c.slots) {
+    if (q.in_flight) {
+      const hipError_t e = hipEventQuery(q.done);
+      if (e == hipErrorNotReady) continue;
+      if (e != hipSuccess)
+        return fail(s->h, PGX_E_HIP, "serving iteration: %s", hipGetErrorString(e));
+      q.in_flight = false;
+      apply(s, q);
+    }
+    if (!free_slot) free_slot = &q;
   }
   complete(s, c);
+  if (!free_slot) return PGX_OK;
   uint32_t staged = 0;
-  int rc = admit(s, c, &staged);
+  int rc = admit(s, c, *free_slot, &staged);
   if (rc != PGX_OK) return rc;
-  return launch_iteration(s, c, staged > 0);
+  return launch_iteration(s, c, *free_slot, staged > 0);
 }
 
 }  // namespace
@@ -349,7 +379,8 @@ int pgx_serve_create(pgx_server_t** out, int device, const pgx_serve_config_t* c
   if (cfg->chunk == 0 || cfg->max_requests == 0 || cfg->pool_columns == 0 ||
       cfg->max_side == 0 || cfg->max_side * 2ull > cfg->pool_columns ||
       (cfg->group_max_items && cfg->group_lanes != 2 && cfg->group_lanes != 4) ||
-      cfg->wide_cu_percent > 99 || (cfg->wide_cu_percent && !cfg->group_min_bits))
+      cfg->wide_cu_percent > 99 || (cfg->wide_cu_percent && !cfg->group_min_bits) ||
+      cfg->slots_per_class > 4)
     return fail(nullptr, PGX_E_ARG, "bad serving config");
   const uint32_t base2 = 2;
   pgx_runner_t* h = nullptr;
@@ -442,7 +473,7 @@ int pgx_serve_stats(const pgx_server_t* s, pgx_serve_stats_t* st) {
   for (auto& kv : s->classes) {
     st->active += (uint32_t)kv.second->active.size();
     st->waiting += (uint32_t)kv.second->waiting.size();
-    st->in_flight += kv.second->in_flight ? 1u : 0u;
+    for (const Slot& q : kv.second->slots) st->in_flight += q.in_flight ? 1u : 0u;
   }
   return PGX_OK;
 }
@@ -451,7 +482,8 @@ int pgx_serve_destroy(pgx_server_t* s) {
   if (!s) return PGX_E_ARG;
   (void)hipSetDevice(s->h->device);
   for (auto& kv : s->classes) {
-    if (kv.second->stream) (void)hipStreamSynchronize(kv.second->stream);
+    for (Slot& q : kv.second->slots)
+      if (q.stream) (void)hipStreamSynchronize(q.stream);
     release_class(*kv.second);
   }
   pgx_runner_destroy(s->h);

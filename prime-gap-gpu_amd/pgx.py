@@ -112,7 +112,8 @@ class ServeConfig(ctypes.Structure):
                 ("max_requests", ctypes.c_uint32), ("pool_columns", ctypes.c_uint32),
                 ("max_side", ctypes.c_uint32), ("stage_columns", ctypes.c_uint32),
                 ("group_min_bits", ctypes.c_uint32), ("group_max_items", ctypes.c_uint32),
-                ("group_lanes", ctypes.c_uint32), ("wide_cu_percent", ctypes.c_uint32)]
+                ("group_lanes", ctypes.c_uint32), ("wide_cu_percent", ctypes.c_uint32),
+                ("slots_per_class", ctypes.c_uint32)]
 
 
 class ServeRequest(ctypes.Structure):
@@ -631,9 +632,10 @@ class Server:
     def __init__(self, hbm_budget: int = 64 << 30, chunk: int = 32, max_requests: int = 8192,
                  pool_columns: int = 1 << 21, max_side: int = 512, stage_columns: int = 1 << 17,
                  group_min_bits: int = 0, group_max_items: int = 0, group_lanes: int = 4,
-                 wide_cu_percent: int = 0, device: int = 0):
+                 wide_cu_percent: int = 0, slots_per_class: int = 1, device: int = 0):
         cfg = ServeConfig(hbm_budget, chunk, max_requests, pool_columns, max_side, stage_columns,
-                          group_min_bits, group_max_items, group_lanes, wide_cu_percent)
+                          group_min_bits, group_max_items, group_lanes, wide_cu_percent,
+                          slots_per_class)
         h = ctypes.c_void_p()
         rc = lib.pgx_serve_create(ctypes.byref(h), device, ctypes.byref(cfg))
         if rc != PGX_OK:

@@ -60,14 +60,16 @@ def drain(srv, want, timeout=60.0):
     return got
 
 
-@pytest.mark.parametrize("group", [(0, 0, 4, 0), (400, 1 << 20, 4, 0), (400, 1 << 20, 2, 0),
-                                   (600, 1 << 20, 4, 75)])
+@pytest.mark.parametrize("group", [(0, 0, 4, 0, 1), (400, 1 << 20, 4, 0, 1),
+                                   (400, 1 << 20, 2, 0, 1), (600, 1 << 20, 4, 75, 1),
+                                   (0, 0, 4, 0, 2), (600, 1 << 20, 4, 88, 2)])
 def test_alone_and_mixed_equal_oracle(requests, expected, group):
-    """Also with the wide classes' iterations on 2- or 4-lane groups, and with
-    the CUs split between wide and narrow classes."""
-    gb, gi, gl, wide = group
+    """Also with the wide classes' iterations on 2- or 4-lane groups, with the
+    CUs split between wide and narrow classes, and with two iterations of a
+    class in flight at once."""
+    gb, gi, gl, wide, slots = group
     with pgx.Server(chunk=16, group_min_bits=gb, group_max_items=gi, group_lanes=gl,
-                    wide_cu_percent=wide) as srv:
+                    wide_cu_percent=wide, slots_per_class=slots) as srv:
         alone = {}
         for i, (P, words, nxt, prv) in enumerate(requests):
             srv.submit(i, nxt, prv, bits(P))
@@ -89,10 +91,12 @@ def test_alone_and_mixed_equal_oracle(requests, expected, group):
         assert alone[i][2] <= bound
 
 
-def test_budget_forces_queueing(requests, expected):
+@pytest.mark.parametrize("slots", [1, 2])
+def test_budget_forces_queueing(requests, expected, slots):
     # room for about two 1263-bit requests at a time
     budget = 2 * 1000 * (41 + 1) * 4
-    with pgx.Server(hbm_budget=budget, chunk=8, stage_columns=2048) as srv:
+    with pgx.Server(hbm_budget=budget, chunk=8, stage_columns=2048,
+                    slots_per_class=slots) as srv:
         for i, (P, words, nxt, prv) in enumerate(requests):
             srv.submit(i, nxt, prv, bits(P))
         assert srv.stats()["waiting"] > 0
@@ -110,6 +114,8 @@ def test_rejections_and_empty_request(requests):
         pgx.Server(wide_cu_percent=50)
     with pytest.raises(pgx.PgxError):
         pgx.Server(group_min_bits=1000, wide_cu_percent=100)
+    with pytest.raises(pgx.PgxError):
+        pgx.Server(slots_per_class=5)
     P, words, nxt, prv = requests[0]
     with pgx.Server(hbm_budget=1000, max_side=512) as srv:
         with pytest.raises(pgx.PgxError) as e:  # needs more than the budget
