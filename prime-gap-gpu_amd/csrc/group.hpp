@@ -11,10 +11,11 @@
 // the group: for each limb a_i (broadcast from its owner) every lane adds
 // a_i*b_j and m_i*n_j into its 64-bit accumulator slots, the retired position
 // i leaves lane 0, and the window moves down one limb (the slot entering a
-// lane's top comes from the next lane).  Accumulator slots are not normalized
-// during the product: each position takes at most 2*Sp products below 2^56,
-// so Sp <= 127 keeps them below 2^64.  The result is then normalized with G-1
-// rounds of carry passing between the lanes.
+// lane's top comes from the next lane: only its low 28 bits, the carry above
+// them stays in the lane).  Accumulator slots are not normalized during the
+// product: each position takes at most 2*Sp products below 2^56 and one carry
+// below 2^37, so Sp <= 127 keeps them below 2^64.  The result is then
+// normalized with G-1 rounds of carry passing between the lanes.
 //
 // Values stay lazy as in the single-lane kernels (mont.hpp): R = 2^(28*Sp) >=
 // 16N, a product of inputs < 4N is < 2N.  The verdict logic is the one of
@@ -161,31 +162,37 @@ __device__ __forceinline__ void settle_u(uint32_t (&r)[Q], const uint64_t (&t)[Q
 
 // One position of the product scan: add a_I * b and m_I * n, retire
 // position I from lane 0, move the window down one limb.  I is a template
-// argument so the owner lane of a_I and its slot are constants.
+// argument so the owner lane of a_I and its slot are constants.  Every lane
+// splits its lowest slot (position I + lane*Q) at bit RB: the carry stays in
+// the lane, added to the next slot, and only the low RB bits move to the top
+// slot of the lane below (one 32-bit DPP move; lane 0's low bits are zero by
+// the choice of m and leave the window).  top_keep is 0 in the top lane of the
+// group, which has nothing above it, and all ones elsewhere.
 template <int Q, int G, int I>
 __device__ __forceinline__ void scan_step(uint64_t (&acc)[Q], const uint32_t (&a)[Q],
                                           const uint32_t (&b)[Q], const uint32_t (&n)[Q],
-                                          uint32_t ninv, uint32_t lane) {
+                                          uint32_t ninv, uint32_t top_keep) {
+  static_assert(Q > 1, "the carry of the lowest slot stays in the lane");
   const uint32_t ai = bcast<G, I / Q>(a[I % Q]);
 #pragma unroll
   for (int s = 0; s < Q; ++s) acc[s] += (uint64_t)ai * b[s];
   const uint32_t m = bcast<G, 0>(((uint32_t)acc[0] * ninv) & MASK);
 #pragma unroll
   for (int s = 0; s < Q; ++s) acc[s] += (uint64_t)m * n[s];
-  const uint64_t c0 = acc[0] >> RB;  // lane 0: the retired position's carry
-  uint64_t nx = from_next64<G>(acc[0]);
-  if (lane == G - 1) nx = 0;
+  const uint32_t lo = (uint32_t)acc[0] & MASK;
+  acc[1] += acc[0] >> RB;
+  const uint32_t nx = from_next<G>(lo) & top_keep;
 #pragma unroll
   for (int s = 0; s + 1 < Q; ++s) acc[s] = acc[s + 1];
   acc[Q - 1] = nx;
-  if (lane == 0) acc[0] += c0;
 }
 
 template <int Q, int G, int... Is>
 __device__ __forceinline__ void scan_all(std::integer_sequence<int, Is...>, uint64_t (&acc)[Q],
                                          const uint32_t (&a)[Q], const uint32_t (&b)[Q],
-                                         const uint32_t (&n)[Q], uint32_t ninv, uint32_t lane) {
-  (scan_step<Q, G, Is>(acc, a, b, n, ninv, lane), ...);
+                                         const uint32_t (&n)[Q], uint32_t ninv,
+                                         uint32_t top_keep) {
+  (scan_step<Q, G, Is>(acc, a, b, n, ninv, top_keep), ...);
 }
 
 // r <- a * b / R mod N (a, b < 4N with normalized limbs; r < 2N).  r may
@@ -197,7 +204,8 @@ __device__ __forceinline__ void mont_mul(uint32_t (&r)[Q], const uint32_t (&a)[Q
   uint64_t acc[Q];
 #pragma unroll
   for (int s = 0; s < Q; ++s) acc[s] = 0;
-  scan_all<Q, G>(std::make_integer_sequence<int, Q * G>{}, acc, a, b, n, ninv, lane_id<G>());
+  const uint32_t top_keep = (lane_id<G>() == G - 1) ? 0u : ~0u;
+  scan_all<Q, G>(std::make_integer_sequence<int, Q * G>{}, acc, a, b, n, ninv, top_keep);
   settle_u<Q, G>(r, acc);
 }
 
