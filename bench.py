@@ -109,7 +109,7 @@ MIXED = [(211, 16), (337, 16), (503, 24), (701, 32)]  # (P, buffer words), d = 2
 # counter reading of the 256 MiB device copy in the same run (copy read
 # 262144 KiB, FETCH_SIZE read 131087.4; copy wrote 262144 KiB, WRITE_SIZE 262144).
 TRAFFIC_KIB = {
-    "k_prp_plain<17,true>": {"fetch": 2188.41, "fetch_scale": 262144 / 131087.38,
+    "k_prp_plain<17,true>": {"fetch": 2185.43, "fetch_scale": 262144 / 131087.38,
                              "write": 4352.0, "write_scale": 1.0},
 }
 
