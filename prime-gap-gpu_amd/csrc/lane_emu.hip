@@ -25,7 +25,7 @@ uint32_t one_candidate(const uint32_t* x, uint32_t limbs, uint32_t mode, const p
   uint32_t v = PGX_VERDICT_INVALID;
   if (spill == 0) {
     auto word = [&](int j) { return x[j]; };
-    if (mode == (PGX_MODE_FERMAT | kColumnShift))  // the plain kernels' form at 12..24 limbs
+    if (mode == (PGX_MODE_FERMAT | kColumnShift))  // the squaring with column sums shifted
       v = pgx::prp_candidate<S, true, false>(w, word, PGX_MODE_FERMAT, bases, rw);
     else if (mode == PGX_MODE_FERMAT)
       v = pgx::prp_candidate<S, true>(w, word, mode, bases, rw);
