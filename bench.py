@@ -309,7 +309,7 @@ def bench_1024(stream, peak: float, reps: int) -> dict:
 def bench_2048(stream, peak: float, reps: int) -> dict:
     """The 2048-bit class: P=1447, d=210 (~2020-bit N in 64-word buffers),
     2^16 sieved candidates, Fermat + residues: the plain one-lane launch
-    against groups of 4 lanes per candidate, and the autotuner's pick."""
+    against groups of 4 and 8 lanes per candidate, and the autotuner's pick."""
     import pgx
     L = gen(1447, 210, 64, 16, 4096, SEED + 1447)
     a = pgx.DeviceBatch(L)
@@ -322,6 +322,10 @@ def bench_2048(stream, peak: float, reps: int) -> dict:
     def lanes4():
         run.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True,
                             tuning=pgx.Tuning(8, 4, 64, 4), stream=stream.cuda_stream)
+
+    def lanes8():
+        run.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True,
+                            tuning=pgx.Tuning(8, 4, 64, 8), stream=stream.cuda_stream)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     ov, ores = oracle.verdicts(L[:256], oracle.MODE_FERMAT, residues=True)
@@ -329,7 +333,7 @@ def bench_2048(stream, peak: float, reps: int) -> dict:
                        "Fermat + residues", "bits": median_bits(L)}
     w = work_per_test(out["bits"]) * a.n
     ok = True
-    for name, fn in (("plain", plain), ("lanes4", lanes4)):
+    for name, fn in (("plain", plain), ("lanes4", lanes4), ("lanes8", lanes8)):
         fn()
         ms = time_ms(stream, fn, reps)
         ok = ok and bool(np.array_equal(a.verdicts()[:256], ov) and
@@ -761,7 +765,8 @@ def main() -> int:
         extra["class_2048"] = bench_2048(stream, peak, max(2, args.steps // 4))
         c = extra["class_2048"]
         log(f"2048-bit: plain {c['plain_tests_per_s'] / 1e6:.3f} M/s, 4 lanes "
-            f"{c['lanes4_tests_per_s'] / 1e6:.3f} M/s, tuned {c['tuned_tests_per_s'] / 1e6:.3f} M/s "
+            f"{c['lanes4_tests_per_s'] / 1e6:.3f} M/s, 8 lanes {c['lanes8_tests_per_s'] / 1e6:.3f} "
+            f"M/s, tuned {c['tuned_tests_per_s'] / 1e6:.3f} M/s "
             f"{c['tuned']}, cpu {c['cpu']['tests_per_s']:.0f}/s")
     if "mixed" in todo:
         extra["mixed_graph"] = bench_mixed(stream, peak, max(2, args.steps // 4))

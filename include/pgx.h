@@ -186,8 +186,9 @@ typedef struct {
   uint32_t waves_per_block;  /* wavefronts per workgroup, 1..4 */
   uint32_t chunk;            /* candidates a wavefront takes per counter bump, multiple of 64 */
   uint32_t lanes_per_number; /* lanes that share one candidate: 1 (persistent workgroups,
-                                the fields above apply), 2 or 4 (a lane group per candidate,
-                                one work item per group: for launches with few candidates) */
+                                the fields above apply), 2, 4 or 8 (a lane group per
+                                candidate, one work item per group: for launches with few
+                                candidates) */
 } pgx_tuning_t;
 
 typedef struct pgx_plan pgx_plan_t;
@@ -286,7 +287,7 @@ typedef struct {
   uint32_t stage_columns;/* columns admitted per class per iteration (pinned staging) */
   uint32_t group_min_bits;  /* classes at least this wide (kernel capacity, bits) test ... */
   uint32_t group_max_items; /* ... iterations of at most this many candidates with ... */
-  uint32_t group_lanes;     /* ... 2 or 4 lanes per candidate (lower latency for the wide
+  uint32_t group_lanes;     /* ... 2, 4 or 8 lanes per candidate (lower latency for the wide
                                classes on a lightly loaded GPU); group_max_items 0 = never */
   uint32_t wide_cu_percent; /* 1..99: the classes at least group_min_bits wide run on this
                                share of the CUs and the narrower ones on the rest, so a

@@ -1,5 +1,5 @@
 // Several lanes per candidate: the PRP test of one number spread over a group
-// of G = 2 or 4 consecutive lanes (inside one quad), for launches with few
+// of G = 2, 4 or 8 consecutive lanes (inside one row of 16), for launches with few
 // candidates, where one lane per candidate leaves most SIMDs idle and the
 // latency of a single exponentiation bounds the launch (Miller-Rabin bases on
 // a few survivors, serving iterations of wide classes).
@@ -7,7 +7,8 @@
 // A number is Sp = Q*G radix-2^28 limbs; lane r of a group owns limbs
 // r*Q .. r*Q+Q-1 in registers.  Cross-lane traffic is DPP quad permutations
 // (no LDS): broadcasts inside the group, and moves to the next / previous
-// lane.  The Montgomery product is coarsely integrated operand scanning over
+// lane (quad permutations for G <= 4; for G = 8 also row shifts by 4 and 1
+// lanes, whose reads across a group boundary callers mask or select away).  The Montgomery product is coarsely integrated operand scanning over
 // the group: for each limb a_i (broadcast from its owner) every lane adds
 // a_i*b_j and m_i*n_j into its 64-bit accumulator slots, the retired position
 // i leaves lane 0, and the window moves down one limb (the slot entering a
@@ -43,25 +44,52 @@ __device__ __forceinline__ uint32_t dpp(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
 }
 
+// DPP row shifts (lanes of a row of 16; lanes shifted in from outside the
+// row read 0).
+constexpr int kRowShl1 = 0x101, kRowShl4 = 0x104, kRowShr1 = 0x111, kRowShr4 = 0x114;
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_row(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+}
+
+// In groups of 8: the value of the lane 4 above (lanes 0-3 of the group) or 4
+// below (lanes 4-7), i.e. the other quad of the group.
+__device__ __forceinline__ uint32_t other_quad(uint32_t v) {
+  const uint32_t up = dpp_row<kRowShl4>(v), down = dpp_row<kRowShr4>(v);
+  return (threadIdx.x & 4u) ? down : up;
+}
+
 // Value of group lane O (compile-time) in every lane of the group.
 template <int G, int O>
 __device__ __forceinline__ uint32_t bcast(uint32_t v) {
-  static_assert(G == 2 || G == 4, "groups of 2 or 4 lanes");
-  if constexpr (G == 4) return dpp<qp(O, O, O, O)>(v);
-  else return dpp<qp(O, O, 2 + O, 2 + O)>(v);
+  static_assert(G == 2 || G == 4 || G == 8, "groups of 2, 4 or 8 lanes");
+  if constexpr (G == 4) {
+    return dpp<qp(O, O, O, O)>(v);
+  } else if constexpr (G == 2) {
+    return dpp<qp(O, O, 2 + O, 2 + O)>(v);
+  } else {
+    const uint32_t q = dpp<qp(O % 4, O % 4, O % 4, O % 4)>(v);  // lane O % 4 of each quad
+    const uint32_t o = other_quad(q);
+    return ((threadIdx.x & 4u) != 0) == (O >= 4) ? q : o;
+  }
 }
 
-// Value of lane r+1 (the top lane gets its own value back: callers mask it).
+// Value of lane r+1 (the top lane gets its own value, or the next group's, or
+// 0 back: callers mask it).
 template <int G>
 __device__ __forceinline__ uint32_t from_next(uint32_t v) {
-  if constexpr (G == 4) return dpp<qp(1, 2, 3, 3)>(v);
+  if constexpr (G == 8) return dpp_row<kRowShl1>(v);
+  else if constexpr (G == 4) return dpp<qp(1, 2, 3, 3)>(v);
   else return dpp<qp(1, 1, 3, 3)>(v);
 }
 
-// Value of lane r-1 (lane 0 gets its own value back: callers mask it).
+// Value of lane r-1 (lane 0 gets its own value, or the previous group's, or 0
+// back: callers mask it).
 template <int G>
 __device__ __forceinline__ uint32_t from_prev(uint32_t v) {
-  if constexpr (G == 4) return dpp<qp(0, 0, 1, 2)>(v);
+  if constexpr (G == 8) return dpp_row<kRowShr1>(v);
+  else if constexpr (G == 4) return dpp<qp(0, 0, 1, 2)>(v);
   else return dpp<qp(0, 0, 2, 2)>(v);
 }
 
@@ -85,7 +113,8 @@ __device__ __forceinline__ uint64_t dpp64(uint64_t v) {
 template <int G>
 __device__ __forceinline__ uint32_t group_or(uint32_t v) {
   v |= dpp<qp(1, 0, 3, 2)>(v);
-  if constexpr (G == 4) v |= dpp<qp(2, 3, 0, 1)>(v);
+  if constexpr (G >= 4) v |= dpp<qp(2, 3, 0, 1)>(v);
+  if constexpr (G == 8) v |= other_quad(v);
   return v;
 }
 
@@ -93,9 +122,15 @@ template <int G>
 __device__ __forceinline__ double group_sum(double v) {
   v += __longlong_as_double(
       (long long)dpp64<qp(1, 0, 3, 2)>((uint64_t)__double_as_longlong(v)));
-  if constexpr (G == 4)
+  if constexpr (G >= 4)
     v += __longlong_as_double(
         (long long)dpp64<qp(2, 3, 0, 1)>((uint64_t)__double_as_longlong(v)));
+  if constexpr (G == 8) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint64_t o = (uint64_t)other_quad((uint32_t)u) |
+                       ((uint64_t)other_quad((uint32_t)(u >> 32)) << 32);
+    v += __longlong_as_double((long long)o);
+  }
   return v;
 }
 

@@ -1,4 +1,4 @@
-// PRP kernels with G = 2 or 4 lanes per candidate (group.hpp).  Same
+// PRP kernels with G = 2, 4 or 8 lanes per candidate (group.hpp).  Same
 // arguments, modes, verdict words and residues as the one-lane kernels of
 // prp_kernels.hip; work item i is tested by lanes G*i .. G*i+G-1.
 #include "group.hpp"
@@ -132,6 +132,7 @@ struct RegisterGroups {
   RegisterGroups() {
     PrpLaunchFn* t2 = group_table(2);
     PrpLaunchFn* t4 = group_table(4);
+    PrpLaunchFn* t8 = group_table(8);
     t2[16] = &launch_group<8, 2>;
     t2[24] = &launch_group<12, 2>;
     t2[32] = &launch_group<16, 2>;
@@ -145,6 +146,10 @@ struct RegisterGroups {
     t4[56] = &launch_group<14, 4>;
     t4[64] = &launch_group<16, 4>;
     t4[76] = &launch_group<19, 4>;
+    t8[32] = &launch_group<4, 8>;
+    t8[48] = &launch_group<6, 8>;
+    t8[64] = &launch_group<8, 8>;
+    t8[80] = &launch_group<10, 8>;
   }
 } register_groups;
 

@@ -28,7 +28,8 @@ PersistLaunchFn* persist_table() {
 PrpLaunchFn* group_table(int G) {
   static PrpLaunchFn two[kMaxKernelLimbs + 1] = {};
   static PrpLaunchFn four[kMaxKernelLimbs + 1] = {};
-  return G == 2 ? two : four;
+  static PrpLaunchFn eight[kMaxKernelLimbs + 1] = {};
+  return G == 2 ? two : G == 4 ? four : eight;
 }
 
 std::string& create_error() {

@@ -378,7 +378,8 @@ int pgx_serve_create(pgx_server_t** out, int device, const pgx_serve_config_t* c
   *out = nullptr;
   if (cfg->chunk == 0 || cfg->max_requests == 0 || cfg->pool_columns == 0 ||
       cfg->max_side == 0 || cfg->max_side * 2ull > cfg->pool_columns ||
-      (cfg->group_max_items && cfg->group_lanes != 2 && cfg->group_lanes != 4) ||
+      (cfg->group_max_items && cfg->group_lanes != 2 && cfg->group_lanes != 4 &&
+       cfg->group_lanes != 8) ||
       cfg->wide_cu_percent > 99 || (cfg->wide_cu_percent && !cfg->group_min_bits) ||
       cfg->slots_per_class > 4)
     return fail(nullptr, PGX_E_ARG, "bad serving config");

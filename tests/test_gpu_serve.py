@@ -62,9 +62,10 @@ def drain(srv, want, timeout=60.0):
 
 @pytest.mark.parametrize("group", [(0, 0, 4, 0, 1), (400, 1 << 20, 4, 0, 1),
                                    (400, 1 << 20, 2, 0, 1), (600, 1 << 20, 4, 75, 1),
-                                   (0, 0, 4, 0, 2), (600, 1 << 20, 4, 88, 2)])
+                                   (0, 0, 4, 0, 2), (600, 1 << 20, 4, 88, 2),
+                                   (600, 1 << 20, 8, 88, 1)])
 def test_alone_and_mixed_equal_oracle(requests, expected, group):
-    """Also with the wide classes' iterations on 2- or 4-lane groups, with the
+    """Also with the wide classes' iterations on 2-, 4- or 8-lane groups, with the
     CUs split between wide and narrow classes, and with two iterations of a
     class in flight at once."""
     gb, gi, gl, wide, slots = group
