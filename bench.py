@@ -399,7 +399,7 @@ def bench_large(stream, peak: float, reps: int) -> dict:
     pipeline()
     ms_fermat = time_ms(stream, fermat_only, reps)
     by_lanes = {}
-    for lanes in (1, 2, 4):  # lanes per candidate of the Miller-Rabin stage
+    for lanes in (1, 2, 4, 8):  # lanes per candidate of the Miller-Rabin stage
         mr_lanes[0] = lanes
         pipeline()
         by_lanes[lanes] = time_ms(stream, pipeline, reps)
