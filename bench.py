@@ -242,9 +242,10 @@ def bench_plain_streams(cands: np.ndarray, max_bits: int, reps: int, nstreams: i
 
 def bench_host_boundary(cands: np.ndarray, reps: int = 3) -> dict:
     """The headline batches handed over at the C ABI from host memory, as the
-    drop-in does: pgx_run with candidate limbs (transpose into pinned staging,
-    H2D, kernel, D2H of the verdicts) and pgx_submit_mk with the multipliers
-    only (candidates built on the device).  PCIe and host work included."""
+    drop-in does: pgx_run with candidate limbs (memcpy into pinned staging,
+    H2D, device transpose, kernel, D2H of the verdicts) and pgx_submit_mk with
+    the multipliers only (candidates built on the device).  PCIe and host work
+    included.  Then the construction alone: the device builder against GMP."""
     import pgx
     K = pgx.primorial_over(P, D)
     n = len(cands)
@@ -265,6 +266,30 @@ def bench_host_boundary(cands: np.ndarray, reps: int = 3) -> dict:
         out["pgx_submit_mk_tests_per_s"] = reps * (n // BATCH) * BATCH / (time.perf_counter() - t0)
     out["workload"] = ("headline candidates in 65536-candidate batches from host memory, "
                        "synchronous per batch (one runner, one stream)")
+    # construction alone (fill_batch, gpu_testing.cu:136-195): the device
+    # builder on resident buffers against GMP on one host thread
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    kw = pgx.to_limbs([K], (K.bit_length() + 31) // 32)[0]
+    d_k = torch.from_numpy(kw.view(np.int32)).cuda()
+    d_m = torch.from_numpy(mult.view(np.int64)).cuda()
+    d_out = torch.zeros((LIMBS, BATCH), dtype=torch.int32, device="cuda")
+    d_st = torch.zeros(1, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.Stream()
+    with pgx.Runner(1, limbs=LIMBS) as run:
+        def build():
+            run.build_candidates(d_k.data_ptr(), len(kw), d_m.data_ptr(), 0, x, 1, BATCH,
+                                 d_out.data_ptr(), BATCH, LIMBS, d_st.data_ptr(),
+                                 stream.cuda_stream)
+        build()
+        ms = time_ms(stream, build, 50)
+    host = d_out.cpu().numpy().view(np.uint32).T
+    want, cpu_s = oracle.build_mk(kw, mult, x, 1, LIMBS)
+    out["device_build_candidates_per_s"] = BATCH / (ms * 1e-3)
+    out["device_build_ms_per_65536"] = ms
+    out["cpu_build_candidates_per_s_one_thread"] = BATCH / cpu_s
+    out["build_equals_gmp"] = bool(np.array_equal(host, want) and int(d_st.cpu()[0]) == 0)
     return out
 
 
@@ -755,8 +780,12 @@ def main() -> int:
         log(f"plain on 3 streams: {extra['plain_three_streams']['tests_per_s'] / 1e6:.2f} M tests/s")
     if "host" in todo:
         extra["host_boundary"] = bench_host_boundary(cands)
-        log(f"host boundary: pgx_run {extra['host_boundary']['pgx_run_tests_per_s'] / 1e6:.2f} "
-            f"M/s, pgx_submit_mk {extra['host_boundary']['pgx_submit_mk_tests_per_s'] / 1e6:.2f} M/s")
+        hb = extra["host_boundary"]
+        log(f"host boundary: pgx_run {hb['pgx_run_tests_per_s'] / 1e6:.2f} M/s, pgx_submit_mk "
+            f"{hb['pgx_submit_mk_tests_per_s'] / 1e6:.2f} M/s; construction on the device "
+            f"{hb['device_build_candidates_per_s'] / 1e9:.2f} G/s vs GMP one thread "
+            f"{hb['cpu_build_candidates_per_s_one_thread'] / 1e6:.2f} M/s "
+            f"(equal: {hb['build_equals_gmp']})")
     if "1024" in todo:
         extra["class_1024"] = bench_1024(stream, peak, max(2, args.steps // 4))
         log(f"1024-bit: plain {extra['class_1024']['plain_tests_per_s'] / 1e6:.2f} M/s, "

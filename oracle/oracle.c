@@ -387,6 +387,36 @@ double orc_time_powm2(const uint32_t* limbs, size_t n, uint32_t S, int threads, 
   return t1 - t0;
 }
 
+/* ---- candidate construction (fill_batch, gpu_testing.cu:136-195) --------- */
+
+/* out[i] (S words, candidate-major) = m[i]*K + sign*x, with fill_batch's GMP
+ * calls (mpz_mul_ui, then mpz_add_ui or mpz_sub_ui; gpu_testing.cu:171-172)
+ * and from_mpz's packing (mpz_export, miller_rabin.h:39-54).  *seconds gets the
+ * wall time of the loop on one thread.  ORC_E_INPUT if a value does not fit S
+ * words or is negative. */
+int orc_build_mk(const uint32_t* K, uint32_t k_words, const uint64_t* m, size_t n, uint32_t x,
+                 int sign, uint32_t S, uint32_t* out, double* seconds) {
+  if (!K || !k_words || (n && (!m || !out)) || (sign != 1 && sign != -1) || !S)
+    return ORC_E_ARG;
+  mpz_t k, z;
+  mpz_inits(k, z, NULL);
+  to_mpz(k, K, k_words);
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  int rc = ORC_OK;
+  for (size_t i = 0; i < n && rc == ORC_OK; ++i) {
+    mpz_mul_ui(z, k, (unsigned long)m[i]);
+    if (sign > 0) mpz_add_ui(z, z, x);
+    else mpz_sub_ui(z, z, x);
+    if (mpz_sgn(z) < 0) rc = ORC_E_INPUT;
+    else rc = from_mpz(z, out + i * S, S);
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  if (seconds) *seconds = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+  mpz_clears(k, z, NULL);
+  return rc;
+}
+
 /* ---- overflow search (overflow.cpp:142-160) ------------------------------ */
 
 /* Offset X of the next prime above m*K (sign > 0: mpz_nextprime(m*K), the

@@ -55,6 +55,9 @@ def _load():
     lib.orc_time_powm2.restype = ctypes.c_double
     lib.orc_next_prime_offset.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
                                           ctypes.c_uint32, u32p]
+    lib.orc_build_mk.argtypes = [u32p, ctypes.c_uint32, u64p, ctypes.c_size_t, ctypes.c_uint32,
+                                 ctypes.c_int, ctypes.c_uint32, u32p,
+                                 ctypes.POINTER(ctypes.c_double)]
     szp = ctypes.POINTER(ctypes.c_size_t)
     lib.orc_ms_active.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                   ctypes.c_uint32, u32p, szp]
@@ -145,6 +148,20 @@ def time_powm2(limbs: np.ndarray, threads: int, warmup: bool = True):
     cnt = np.zeros(1, dtype=np.uint32)
     sec = lib.orc_time_powm2(_u32(a), a.shape[0], a.shape[1], threads, int(warmup), _u32(cnt))
     return sec, int(cnt[0])
+
+
+def build_mk(K_words, m: np.ndarray, x: int, sign: int, S: int):
+    """(candidate-major limbs [n, S] of m[i]*K + sign*x, seconds on one thread):
+    fill_batch's construction (gpu_testing.cu:171-172) with GMP."""
+    k = np.ascontiguousarray(K_words, dtype=np.uint32)
+    mm = np.ascontiguousarray(m, dtype=np.uint64)
+    out = np.zeros((len(mm), S), dtype=np.uint32)
+    sec = ctypes.c_double(0.0)
+    rc = lib.orc_build_mk(_u32(k), len(k), _u64(mm), len(mm), x, sign, S, _u32(out),
+                          ctypes.byref(sec))
+    if rc != 0:
+        raise ValueError(f"orc_build_mk failed: {rc}")
+    return out, sec.value
 
 
 def next_prime_offset(K_words, m: int, sign: int, limit: int) -> int:

@@ -164,3 +164,22 @@ def test_next_prime_offset_vs_sympy():
         assert O.next_prime_offset(kw, m, 1, 30000) == sympy.nextprime(c) - c
         assert O.next_prime_offset(kw, m, -1, 30000) == c - sympy.prevprime(c)
     assert O.next_prime_offset(kw, 10000019, 1, 10) == 0  # beyond the limit
+
+
+def test_build_mk_vs_python_bignums():
+    """fill_batch's construction in the oracle (mpz_mul_ui, mpz_add_ui /
+    mpz_sub_ui, mpz_export) against CPython integers, both signs, and the
+    errors for values that do not fit or go negative."""
+    import pgx
+    K = pgx.primorial_over(337, 2310)
+    kw = pgx.to_limbs([K], 14)[0]
+    rng = np.random.default_rng(3)
+    m = rng.integers(1, 1 << 40, size=500, dtype=np.uint64)
+    for sign, x in ((1, 449999), (-1, 17)):
+        out, sec = O.build_mk(kw, m, x, sign, 16)
+        want = pgx.to_limbs([int(v) * K + sign * x for v in m], 16)
+        assert np.array_equal(out, want) and sec >= 0
+    with pytest.raises(ValueError):  # 439 + 40 bits do not fit 14 words
+        O.build_mk(kw, m, 1, 1, 14)
+    with pytest.raises(ValueError):  # 0*K - 1 < 0
+        O.build_mk(kw, np.zeros(1, dtype=np.uint64), 1, -1, 16)
