@@ -656,6 +656,27 @@ def cpu_baseline(cands: np.ndarray, target_cpu_s: float = 15.0) -> dict:
     }
 
 
+def summary(plain_512: float, frac_512, extra: dict) -> dict:
+    """The metric's parts side by side: tests/s at 512/1024/2048 bits (plain
+    launches and the batched/persistent pattern), the INT32 VALU fraction, and
+    the serving p99 latency (None where that leg did not run)."""
+    def get(cfg, key):
+        return extra.get(cfg, {}).get(key)
+    return {
+        "tests_per_s_512_plain": plain_512,
+        "tests_per_s_512_persistent": get("same_workload_persistent", "tests_per_s"),
+        "tests_per_s_1024_plain": get("class_1024", "plain_tests_per_s"),
+        "tests_per_s_1024_tuned": get("class_1024", "tuned_tests_per_s"),
+        "tests_per_s_2048_plain": get("class_2048", "plain_tests_per_s"),
+        "tests_per_s_2048_tuned": get("class_2048", "tuned_tests_per_s"),
+        "valu_frac_512": frac_512,
+        "valu_frac_1024": get("class_1024", "plain_valu_frac"),
+        "valu_frac_2048": get("class_2048", "plain_valu_frac"),
+        "serving_p50_ms": get("serving", "p50_ms"),
+        "serving_p99_ms": get("serving", "p99_ms"),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -871,6 +892,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "parity_sample_ok": parity,
+        "summary": summary(tests_per_s, achieved / peak if peak else None, extra),
         "configs": extra,
     }
     print(json.dumps(line), flush=True)
