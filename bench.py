@@ -468,12 +468,14 @@ def bench_large(stream, peak: float, reps: int) -> dict:
     }
 
 
-def bench_search(n_x: int = 8) -> dict:
+def bench_search(tune_sample: np.ndarray, n_x: int = 8) -> dict:
     """The m-interval search on the device: P=337, d=2310, m in [1e9, 1e9+2e7),
     sieve of every X by the primes below 1e8, tests list, base-2 reference
     PRP tests, found bookkeeping, for the first n_x X of the reference's
-    coprime X list.  CPU baseline leg: the oracle's restatement of the
-    reference's sieve (one thread) at the first X."""
+    coprime X list.  The runner's launch shape for 462-bit reference-mode
+    tests is autotuned first on `tune_sample` (candidates of that width), as a
+    user of the library would.  CPU baseline leg: the oracle's restatement of
+    the reference's sieve (one thread) at the first X."""
     import oracle
     import pgx
     P, d, m_start, m_inc, mp = 337, 2310, 10**9, 20_000_000, 100_000_000
@@ -481,6 +483,9 @@ def bench_search(n_x: int = 8) -> dict:
     kw = pgx.to_limbs([K], (K.bit_length() + 31) // 32)[0]
     xs = [int(x) for x in oracle.ms_coprime_x(P, d)[:n_x]]
     run = pgx.Runner(1 << 20, limbs=16)
+    tb = pgx.DeviceBatch(tune_sample)
+    tuned, _ = run.autotune(tb.bucket(), pgx.MODE_REFERENCE)
+    del tb
     t0 = time.perf_counter()
     s = pgx.Search(run, K, P, d, m_start, m_inc, mp)
     t_setup = time.perf_counter() - t0
@@ -512,6 +517,7 @@ def bench_search(n_x: int = 8) -> dict:
     return {
         "workload": f"m-interval search: P={P} d={d} m in [1e9, 1e9+2e7) ({n_active} active m), "
                     f"sieve primes < 1e8, first {n_x} coprime X, base-2 PRP tests",
+        "tuned": tuned.as_dict(),
         "setup_s": t_setup, "active_m": n_active, "tests": tests, "found": found,
         "tests_first_x": n0,
         "sieve_ms_per_x": 1e3 * float(np.mean(sieve_s)),
@@ -823,7 +829,7 @@ def main() -> int:
     if "large" in todo:
         extra["large_fermat_mr"] = bench_large(stream, peak, max(2, args.steps // 4))
     if "search" in todo:
-        extra["search_interval"] = bench_search()
+        extra["search_interval"] = bench_search(cands[:1 << 18])
         log(f"search: sieve {extra['search_interval']['sieve_ms_per_x']:.2f} ms/X, "
             f"tests {extra['search_interval']['test_ms_per_x']:.2f} ms/X")
     if "serving" in todo:

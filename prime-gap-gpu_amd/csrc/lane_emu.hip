@@ -27,7 +27,7 @@ uint32_t one_candidate(const uint32_t* x, uint32_t limbs, uint32_t mode, const p
     auto word = [&](int j) { return x[j]; };
     if (mode == (PGX_MODE_FERMAT | kColumnShift))  // the squaring with column sums shifted
       v = pgx::prp_candidate<S, true, false>(w, word, PGX_MODE_FERMAT, bases, rw);
-    else if (mode == PGX_MODE_FERMAT)
+    else if (mode == PGX_MODE_FERMAT || (bases.count == 1 && bases.a[0] == 2u))  // as launched
       v = pgx::prp_candidate<S, true>(w, word, mode, bases, rw);
     else
       v = pgx::prp_candidate<S, false>(w, word, mode, bases, rw);

@@ -93,10 +93,17 @@ __global__ __launch_bounds__(kBlock) void k_prp_persistent(PrpArgs a, uint32_t* 
   }
 }
 
+// The base-2 kernels serve FERMAT mode and every mode whose base list is {2}
+// (the reference's default, ROUNDS = 1): fewer registers than the general
+// Miller-Rabin kernels, which carry the small-base step.
+inline bool base2_only(const PrpArgs& a) {
+  return a.mode == PGX_MODE_FERMAT || (a.bases.count == 1 && a.bases.a[0] == 2u);
+}
+
 template <int S>
 hipError_t launch_plain(const PrpArgs& a, hipStream_t stream) {
   const uint32_t grid = (a.n + kBlock - 1) / kBlock;
-  if (a.mode == PGX_MODE_FERMAT)
+  if (base2_only(a))
     hipLaunchKernelGGL((k_prp_plain<S, true>), dim3(grid), dim3(kBlock), 0, stream, a);
   else
     hipLaunchKernelGGL((k_prp_plain<S, false>), dim3(grid), dim3(kBlock), 0, stream, a);
@@ -106,7 +113,7 @@ hipError_t launch_plain(const PrpArgs& a, hipStream_t stream) {
 template <int S>
 hipError_t launch_persistent(const PrpArgs& a, uint32_t* counter, const PersistShape& sh,
                              hipStream_t stream) {
-  if (a.mode == PGX_MODE_FERMAT)
+  if (base2_only(a))
     hipLaunchKernelGGL((k_prp_persistent<S, true>), dim3(sh.blocks), dim3(sh.threads), 0, stream,
                        a, counter, sh.chunk);
   else

@@ -211,3 +211,22 @@ def test_bench_refuses_several_gpus():
     r = subprocess.run([_sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_lane_routine_base_two_alone_in_every_mode(emu):
+    """The base list {2} runs the base-2 routine in every mode (as the kernels
+    are launched); its verdict counts equal the general routine's (list
+    {2, 3}, first base) and GMP's."""
+    m = pgx.gen_multipliers(0x5EED0005, 10**7, 10**7 + 10**4, 2310, 2)
+    L, _, _ = pgx.gen_sieved(337, 2310, m, 200000, 300, 16)
+    Ns = list(range(3, 4096, 2))
+    for arr in (L, pgx.to_limbs(Ns, 16)):
+        for mode, omode in ((pgx.MODE_REFERENCE, O.MODE_REFERENCE),
+                            (pgx.MODE_STRONG, O.MODE_STRONG)):
+            v, r = emu(arr, 17, mode, (2,))
+            want = O.verdicts(arr, omode, (2,))
+            assert np.array_equal(v, want), mode
+            v23, _ = emu(arr, 17, mode, (2, 3))
+            assert np.array_equal(np.minimum(v23, 1), v), mode
+            _, ores = O.verdicts(arr, O.MODE_FERMAT, residues=True)
+            assert np.array_equal(r, ores), mode
