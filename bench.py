@@ -293,6 +293,51 @@ def bench_host_boundary(cands: np.ndarray, reps: int = 3) -> dict:
     return out
 
 
+def bench_rounds(cands: np.ndarray, stream, n: int = 1 << 18) -> dict:
+    """Several Miller-Rabin bases (test_runner_t(n, rounds) with rounds > 1,
+    miller_rabin.h:322-349) on the first n headline candidates, reference mode.
+    pgx_run in batches of 65536 (one pass over every base: below two device
+    fills) and of n (the first base on every candidate, the others on the
+    device-compacted list of those that passed it); pgx_test_device one pass
+    over 65536-candidate batches on resident buffers (no PCIe) for scale."""
+    import pgx
+    import torch
+    n = min(n, len(cands))
+    out = {"workload": f"first {n} headline candidates, reference mode"}
+    verdicts = {}
+    for rounds in (1, 2, 7):
+        for batch in (BATCH, n):
+            if rounds == 1 and batch == n:
+                continue
+            with pgx.Runner(batch, rounds=rounds, limbs=LIMBS) as run:
+                run.run(cands[:batch], pgx.MODE_REFERENCE)
+                t0 = time.perf_counter()
+                got = [run.run(cands[c0:c0 + batch], pgx.MODE_REFERENCE)
+                       for c0 in range(0, n, batch)]
+                out[f"pgx_run_rounds{rounds}_batch{batch}_tests_per_s"] = (
+                    n / (time.perf_counter() - t0))
+                verdicts[(rounds, batch)] = np.concatenate(got)
+    batches = [pgx.DeviceBatch(cands[c0:c0 + BATCH], residues=False) for c0 in range(0, n, BATCH)]
+    for rounds in (1, 7):
+        with pgx.Runner(1, rounds=rounds, limbs=LIMBS) as run:
+            def one_pass():
+                for b in batches:
+                    run.test_device(b.cand.data_ptr(), b.ld, LIMBS, b.n, b.max_bits,
+                                    pgx.MODE_REFERENCE, b.verdict.data_ptr(),
+                                    stream=stream.cuda_stream)
+            one_pass()
+            ms = time_ms(stream, one_pass, 2)
+            out[f"one_pass_device_rounds{rounds}_tests_per_s"] = n / (ms * 1e-3)
+        if rounds == 7:
+            torch.cuda.synchronize()
+            one = np.concatenate([b.verdicts() for b in batches])
+            out["rounds7_verdicts_equal"] = bool(
+                np.array_equal(one, verdicts[(7, BATCH)]) and np.array_equal(one, verdicts[(7, n)]))
+    out["passed_first_base"] = int((verdicts[(7, n)] >= 1).sum())
+    out["passed_all_7"] = int((verdicts[(7, n)] == 7).sum())
+    return out
+
+
 def bench_1024(stream, peak: float, reps: int) -> dict:
     """The 1024-bit class alone: P=701, d=210 (987-bit N in 32-word buffers),
     2^18 candidates; plain launches of 65536 vs one autotuned launch."""
@@ -691,7 +736,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline plain-kernel workload")
-    ap.add_argument("--extras", default="persistent,streams,host,1024,2048,mixed,large,search,serving",
+    ap.add_argument("--extras", default="persistent,streams,host,rounds,1024,2048,mixed,large,search,serving",
                     help="comma-separated extra configurations to run")
     ap.add_argument("--serve-group", default="1000,32768,4,88",
                     help="bits,items,lanes[,cu%%[,slots]]: serving iterations of classes at "
@@ -813,6 +858,12 @@ def main() -> int:
             f"{hb['device_build_candidates_per_s'] / 1e9:.2f} G/s vs GMP one thread "
             f"{hb['cpu_build_candidates_per_s_one_thread'] / 1e6:.2f} M/s "
             f"(equal: {hb['build_equals_gmp']})")
+    if "rounds" in todo:
+        extra["rounds"] = bench_rounds(cands, stream)
+        r = extra["rounds"]
+        log("rounds: " + ", ".join(f"{k.replace('_tests_per_s', '')} {v / 1e6:.2f} M/s"
+                                    for k, v in r.items() if k.endswith("_tests_per_s"))
+            + f" (equal: {r['rounds7_verdicts_equal']})")
     if "1024" in todo:
         extra["class_1024"] = bench_1024(stream, peak, max(2, args.steps // 4))
         log(f"1024-bit: plain {extra['class_1024']['plain_tests_per_s'] / 1e6:.2f} M/s, "

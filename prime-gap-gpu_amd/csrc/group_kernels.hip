@@ -82,6 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_prp_group(PrpArgs a) {
     } else {
       const uint32_t need = (a.mode == PGX_MODE_STRONG) ? FLAG_SPRP : FLAG_QUIRK;
       uint32_t passed = 0;
+      bool alive = true;  // every base so far passed
       for (uint32_t b = 0; b < a.bases.count; ++b) {
         uint32_t f;
         if (b == 0) {
@@ -90,8 +91,10 @@ __global__ __launch_bounds__(kBlock) void k_prp_group(PrpArgs a) {
           uint32_t scratch[Q];
           f = group_test_base<Q, G>(a.bases.a[b], n, one, ninv, L, t, inv_n, a, c, scratch);
         }
-        if (!(f & need)) break;
-        ++passed;
+        alive = alive && (f & need);
+        passed += alive ? 1u : 0u;
+        // wavefront-uniform exit, as in the one-lane kernels
+        if (!wave_any(alive)) break;
       }
       v = passed;
     }

@@ -52,6 +52,16 @@ struct Radix {
 
 PGX_FN uint64_t mul64(uint32_t a, uint32_t b) { return (uint64_t)a * b; }
 
+// True when pred holds on some active lane of the wavefront; a host build
+// emulates one lane, whose own pred it returns.
+PGX_FN bool wave_any(bool pred) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __ballot(pred) != 0;
+#else
+  return pred;
+#endif
+}
+
 // -N^-1 mod 2^32 for odd n0 (Newton iteration, 3 -> 6 -> 12 -> 24 -> 48 bits);
 // its low RB bits are -N^-1 mod 2^RB.
 PGX_FN uint32_t neg_inv32(uint32_t n0) {

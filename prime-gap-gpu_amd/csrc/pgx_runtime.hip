@@ -134,6 +134,7 @@ void release(pgx_runner* h) {
   if (h->d_verdict) (void)hipFree(h->d_verdict);
   if (h->d_res) (void)hipFree(h->d_res);
   if (h->d_counters) (void)hipFree(h->d_counters);
+  if (h->d_survivors) (void)hipFree(h->d_survivors);
   if (h->d_K) (void)hipFree(h->d_K);
   if (h->d_m) (void)hipFree(h->d_m);
   if (h->d_status) (void)hipFree(h->d_status);
@@ -143,9 +144,14 @@ void release(pgx_runner* h) {
 
 }  // namespace
 
-int pgx::launch_prp(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs, size_t n,
-                    uint32_t max_bits, uint32_t flags, uint32_t* d_verdict, uint32_t* d_res,
-                    uint32_t res_ld, hipStream_t stream) {
+namespace {
+
+// Plain launch with an explicit base list, over columns 0..n-1 or, with index
+// and count, over the work list index[0..min(*count, n)).
+int launch_plain_bases(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs,
+                       size_t n, uint32_t max_bits, uint32_t flags, const pgx::Bases& bases,
+                       const uint32_t* index, const uint32_t* count, uint32_t* d_verdict,
+                       uint32_t* d_res, uint32_t res_ld, hipStream_t stream) {
   const int S = pgx::kernel_for_bits(max_bits);
   if (S == 0)
     return fail(h, PGX_E_INPUT, "candidates of %u bits exceed the widest kernel (%d bits)",
@@ -157,12 +163,60 @@ int pgx::launch_prp(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t
   a.n = (uint32_t)n;
   a.buf_limbs = limbs;
   a.mode = flags & PGX_MODE_MASK;
-  a.bases = h->bases;
+  a.bases = bases;
+  a.index = index;
+  a.count = count;
   a.verdict = d_verdict;
   a.residue = (flags & PGX_WANT_RESIDUE) ? d_res : nullptr;
   a.res_ld = res_ld;
   PGX_HIP(h, fn(a, stream));
   return PGX_OK;
+}
+
+// The runner's own batch in d_cand (pgx_submit, pgx_submit_mk).  With several
+// bases in a Miller-Rabin mode and a batch of at least two device fills (one
+// fill = one wavefront on every SIMD: n_cu * 4 * 64 candidates), the first
+// base runs on every candidate and the others only on the candidates that
+// passed it, listed on the device.  In one pass every wavefront holding a
+// probable prime carries all the bases, and sieved candidates put a few in
+// nearly every wavefront; the survivors (a few percent) fit in fewer fills.
+// Below two fills both forms take one exponentiation's latency per base and
+// the single pass saves the extra launches.  The verdict stays the count of
+// leading bases passed: the second launch writes the count over the other
+// bases and the bump adds one for the first.  Residues (those of the first
+// base) come from the first launch alone.
+int launch_batch(pgx_runner* h, size_t n, uint32_t max_bits, uint32_t flags) {
+  const uint32_t mode = flags & PGX_MODE_MASK;
+  const size_t fill = (size_t)h->n_cu * 4 * 64;
+  if (h->bases.count == 1 || mode == PGX_MODE_FERMAT || n < 2 * fill)
+    return launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, h->bases,
+                              nullptr, nullptr, h->d_verdict, h->d_res, (uint32_t)n, h->stream);
+  pgx::Bases first{}, rest{};
+  first.count = 1;
+  first.a[0] = h->bases.a[0];
+  rest.count = h->bases.count - 1;
+  for (uint32_t k = 1; k < h->bases.count; ++k) rest.a[k - 1] = h->bases.a[k];
+  uint32_t* count = h->d_counters;  // slot 0: persistent launches use slots 1..
+  int rc = launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, first,
+                              nullptr, nullptr, h->d_verdict, h->d_res, (uint32_t)n, h->stream);
+  if (rc != PGX_OK) return rc;
+  PGX_HIP(h, pgx::launch_compact(h->d_verdict, (uint32_t)n, 1u, h->d_survivors, count,
+                                 h->stream));
+  rc = launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits,
+                          flags & ~PGX_WANT_RESIDUE, rest, h->d_survivors, count, h->d_verdict,
+                          nullptr, 0, h->stream);
+  if (rc != PGX_OK) return rc;
+  PGX_HIP(h, pgx::launch_bump(h->d_verdict, h->d_survivors, count, (uint32_t)n, 1u, h->stream));
+  return PGX_OK;
+}
+
+}  // namespace
+
+int pgx::launch_prp(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs, size_t n,
+                    uint32_t max_bits, uint32_t flags, uint32_t* d_verdict, uint32_t* d_res,
+                    uint32_t res_ld, hipStream_t stream) {
+  return launch_plain_bases(h, d_cand, ld, limbs, n, max_bits, flags, h->bases, nullptr, nullptr,
+                            d_verdict, d_res, res_ld, stream);
 }
 
 using pgx::launch_prp;
@@ -210,6 +264,7 @@ int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t lim
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_verdict, max_n * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_res, words * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_counters, pgx::kCounters * 4);
+  if (e == hipSuccess && nbases > 1) e = hipMalloc((void**)&h->d_survivors, max_n * 4);
   if (e == hipSuccess)
     e = hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -264,8 +319,7 @@ int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t fla
   PGX_HIP(h, hipMemcpyAsync(h->d_res, h->h_cand, (size_t)S * n * 4, hipMemcpyHostToDevice,
                             h->stream));
   PGX_HIP(h, launch_transpose(true, h->d_res, h->d_cand, n, S, (uint32_t)n, h->stream));
-  int rc = launch_prp(h, h->d_cand, (uint32_t)n, S, n, max_bits, flags, h->d_verdict, h->d_res,
-                  (uint32_t)n, h->stream);
+  int rc = launch_batch(h, n, max_bits, flags);
   if (rc != PGX_OK) return rc;
   rc = queue_results(h, n, flags);
   if (rc != PGX_OK) return rc;
@@ -318,8 +372,7 @@ int pgx_submit_mk(pgx_runner_t* h, const uint32_t* K, uint32_t k_words, const ui
   int rc = pgx_build_candidates(h, h->d_K, k_words, h->d_m, nullptr, x, sign, (uint32_t)n,
                                 h->d_cand, (uint32_t)n, h->limbs, h->d_status, h->stream);
   if (rc != PGX_OK) return rc;
-  rc = launch_prp(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, h->d_verdict, h->d_res,
-              (uint32_t)n, h->stream);
+  rc = launch_batch(h, n, max_bits, flags);
   if (rc != PGX_OK) return rc;
   PGX_HIP(h, hipMemcpyAsync(h->h_status, h->d_status, 4, hipMemcpyDeviceToHost, h->stream));
   rc = queue_results(h, n, flags);

@@ -177,6 +177,7 @@ PGX_FN uint32_t prp_candidate(const uint32_t (&w)[Radix<S>::W32], WordFn word, u
     } else {
       const uint32_t need = (mode == PGX_MODE_STRONG) ? FLAG_SPRP : FLAG_QUIRK;
       uint32_t passed = 0;
+      bool alive = true;  // every base so far passed
       for (uint32_t b = 0; b < bases.count; ++b) {
         uint32_t f;
         if (b == 0) {
@@ -186,8 +187,13 @@ PGX_FN uint32_t prp_candidate(const uint32_t (&w)[Radix<S>::W32], WordFn word, u
           uint32_t scratch[S];
           f = test_base<S>(bases.a[b], n, ninv, L, t, word, scratch);
         }
-        if (!(f & need)) break;
-        ++passed;
+        alive = alive && (f & need);
+        passed += alive ? 1u : 0u;
+        // The loop ends for the whole wavefront at once: lanes that failed a
+        // base run the later ones too and drop the results.  A per-lane exit
+        // made the later exponentiations divergent regions, measured 3.7x
+        // slower each than the same exponentiation with every lane active.
+        if (!wave_any(alive)) break;
       }
       v = passed;
     }

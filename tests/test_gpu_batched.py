@@ -128,6 +128,75 @@ def test_indexed_with_empty_list():
     assert np.all(b.verdicts() == 5)
 
 
+# strong pseudoprimes to the first k primes for growing k (2047: base 2;
+# 1373653: 2, 3; ... 3825123056546413051: 2..23), base-3 ones (121, 703,
+# 1891) and Carmichael numbers: candidates that pass the first bases of a list
+# and fail a later one
+PSEUDOPRIMES = [2047, 1373653, 25326001, 3215031751, 2152302898747, 3474749660383,
+                341550071728321, 3825123056546413051, 121, 703, 1891, 561, 1105, 41041]
+
+
+def two_fills():
+    """Batch size from which pgx_submit compacts the first base's survivors:
+    two wavefronts of 64 on each of the 4 SIMDs of every compute unit."""
+    import torch
+    return 2 * 4 * 64 * torch.cuda.get_device_properties(0).multi_processor_count
+
+
+def one_pass(L, bases, mode):
+    """Verdicts of a single launch over every base (pgx_test_device)."""
+    import torch
+    b = pgx.DeviceBatch(L, residues=False)
+    with pgx.Runner(1, limbs=L.shape[1], bases=bases) as r:
+        r.test_device(b.cand.data_ptr(), b.ld, b.limbs, b.n, b.max_bits, mode,
+                      b.verdict.data_ptr())
+        torch.cuda.synchronize()
+    return b.verdicts()
+
+
+@pytest.mark.parametrize("bases", [[2, 3, 5, 7, 11, 13, 17], [3, 2, 5], [2, 3]])
+def test_runner_runs_extra_bases_on_survivors(bases):
+    """pgx_submit with several bases and a batch of two device fills or more
+    tests the first base on every candidate and the others on the
+    device-compacted list of those that passed it.  The verdicts equal a single
+    pass over all bases (pgx_test_device) on every column and the oracle's on
+    the pseudoprimes and a sample, with the first base's residues; the list is
+    rebuilt for the next batch, for a batch with no survivors and for
+    candidates built on the device; a batch below two fills takes the single
+    pass and agrees as well."""
+    fills = two_fills()
+    P = pgx.to_limbs(PSEUDOPRIMES, 16)
+    sieved = gen(337, 2310, 16, 64, fills // 64 + 64)
+    L = np.concatenate([P, sieved[:fills + 100], P])
+    n = len(L)
+    sample = np.r_[0:len(P) + 1500, len(P) + 1500:n - len(P):97, n - len(P):n]
+    with pgx.Runner(n + 4096, limbs=16, bases=bases) as run:
+        for mode in (pgx.MODE_STRONG, pgx.MODE_REFERENCE):
+            say(f"bases {bases} mode {mode}: {n} candidates (two fills = {fills})")
+            v, r = run.run(L, mode, residues=True)
+            assert np.array_equal(v, one_pass(L, bases, mode))
+            ov, ores = O.verdicts(L[sample], mode, tuple(bases), residues=True)
+            assert np.array_equal(v[sample], ov) and np.array_equal(r[sample], ores)
+            assert int((v >= 1).sum()) > int((v == len(bases)).sum())  # later bases fail some
+        say(f"bases {bases}: next batch, below two fills, no survivors")
+        L2 = gen(337, 2310, 16, 64, fills // 64 + 64, seed_off=1)[:fills + 3000]
+        assert np.array_equal(run.run(L2, pgx.MODE_STRONG), one_pass(L2, bases, pgx.MODE_STRONG))
+        small = L2[:4096]
+        assert np.array_equal(run.run(small, pgx.MODE_STRONG),
+                              O.verdicts(small, O.MODE_STRONG, tuple(bases)))
+        none = np.tile(pgx.to_limbs([9, 15, 21, 25, 27, 33], 16), (fills // 6 + 1, 1))
+        assert np.all(run.run(none, pgx.MODE_STRONG) == 0)
+        say(f"bases {bases}: candidates built on the device")
+        K = pgx.primorial_over(337, 2310)
+        m = np.asarray(pgx.gen_multipliers(0x5EED0300, 10**7, 10**7 + 10**6, 2310, fills + 50),
+                       dtype=np.uint64)
+        v = run.run_mk(K, m, 7920, 1, pgx.MODE_STRONG)
+        Lm = pgx.to_limbs([int(mm) * K + 7920 for mm in m], 16)
+        odd = (Lm[:, 0] & 1) == 1
+        assert np.all(v[~odd] == pgx.VERDICT_INVALID)
+        assert np.array_equal(v[odd], one_pass(Lm[odd], bases, pgx.MODE_STRONG))
+
+
 def test_autotune_picks_a_valid_shape():
     import torch
     L = gen(337, 2310, 16, 16, 1024)
