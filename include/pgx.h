@@ -59,10 +59,10 @@ int pgx_runner_destroy(pgx_runner_t* h);
  * send them to the device as they are (a device kernel makes them
  * limb-interleaved) and launch asynchronously on the runner's stream.  The
  * caller's array may be reused as soon as this returns.  With several bases
- * in the reference or strong mode and n of at least 2 * 256 * (compute units),
- * the first base runs on every candidate and the others on the candidates
- * that passed it (compacted on the device); smaller batches take one pass over
- * all bases, as pgx_test_device does.  The verdicts are the same.  Rejects even N,
+ * in the reference or strong mode the first base runs on every candidate and
+ * the others, side by side, on the candidates that passed it (compacted on
+ * the device); pgx_test_device runs every base in one pass.  The verdicts are
+ * the same.  Rejects even N,
  * N <= 1 and N wider than PGX_MAX_BITS with PGX_E_INPUT (the reference
  * asserts / exits, miller_rabin.h:46-49, 375-376). */
 int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t flags);

@@ -35,6 +35,11 @@ struct PersistShape {
 };
 
 typedef hipError_t (*PrpLaunchFn)(const PrpArgs&, hipStream_t);
+// Base-parallel launch over a device-resident work list (index, count): with
+// c = min(*count, n), work item i < c * bases.count tests base
+// bases.a[i / c] on column index[i % c] and sets bit i / c of mask[i % c]
+// when the candidate passes it (mask cleared by the caller).
+typedef hipError_t (*BasesLaunchFn)(const PrpArgs&, uint32_t* mask, hipStream_t);
 typedef hipError_t (*PersistLaunchFn)(const PrpArgs&, uint32_t* counter, const PersistShape&,
                                       hipStream_t);
 
@@ -47,6 +52,7 @@ inline int kernel_capacity_bits(int S) { return (S <= 31 ? 29 : 28) * S - 4; }
 // null when no translation unit instantiated that S.
 PrpLaunchFn* prp_table();
 PersistLaunchFn* persist_table();
+BasesLaunchFn* bases_table();
 
 // Smallest instantiated S whose capacity holds max_bits, or 0.
 inline int kernel_for_bits(uint32_t max_bits) {

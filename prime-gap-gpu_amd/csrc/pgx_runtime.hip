@@ -25,6 +25,11 @@ PersistLaunchFn* persist_table() {
   return table;
 }
 
+BasesLaunchFn* bases_table() {
+  static BasesLaunchFn table[kMaxKernelLimbs + 1] = {};
+  return table;
+}
+
 PrpLaunchFn* group_table(int G) {
   static PrpLaunchFn two[kMaxKernelLimbs + 1] = {};
   static PrpLaunchFn four[kMaxKernelLimbs + 1] = {};
@@ -174,21 +179,18 @@ int launch_plain_bases(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint3
 }
 
 // The runner's own batch in d_cand (pgx_submit, pgx_submit_mk).  With several
-// bases in a Miller-Rabin mode and a batch of at least two device fills (one
-// fill = one wavefront on every SIMD: n_cu * 4 * 64 candidates), the first
-// base runs on every candidate and the others only on the candidates that
-// passed it, listed on the device.  In one pass every wavefront holding a
-// probable prime carries all the bases, and sieved candidates put a few in
-// nearly every wavefront; the survivors (a few percent) fit in fewer fills.
-// Below two fills both forms take one exponentiation's latency per base and
-// the single pass saves the extra launches.  The verdict stays the count of
-// leading bases passed: the second launch writes the count over the other
-// bases and the bump adds one for the first.  Residues (those of the first
-// base) come from the first launch alone.
+// bases in a Miller-Rabin mode: the first base on every candidate; compaction
+// of the columns that passed it; every other base on every survivor at once,
+// one (survivor, base) pair per lane, into a pass mask per survivor; then the
+// verdicts of the survivors from their masks.  In one pass over all bases
+// every wavefront holding a probable prime (sieved candidates put a few in
+// nearly every wavefront) runs the bases one after another; here the later
+// bases cost one exponentiation's latency together.  The verdict stays the
+// count of leading bases passed; residues (the first base's) come from the
+// first launch.
 int launch_batch(pgx_runner* h, size_t n, uint32_t max_bits, uint32_t flags) {
   const uint32_t mode = flags & PGX_MODE_MASK;
-  const size_t fill = (size_t)h->n_cu * 4 * 64;
-  if (h->bases.count == 1 || mode == PGX_MODE_FERMAT || n < 2 * fill)
+  if (h->bases.count == 1 || mode == PGX_MODE_FERMAT)
     return launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, h->bases,
                               nullptr, nullptr, h->d_verdict, h->d_res, (uint32_t)n, h->stream);
   pgx::Bases first{}, rest{};
@@ -197,16 +199,26 @@ int launch_batch(pgx_runner* h, size_t n, uint32_t max_bits, uint32_t flags) {
   rest.count = h->bases.count - 1;
   for (uint32_t k = 1; k < h->bases.count; ++k) rest.a[k - 1] = h->bases.a[k];
   uint32_t* count = h->d_counters;  // slot 0: persistent launches use slots 1..
+  uint32_t* mask = h->d_survivors + h->max_n;
   int rc = launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, first,
                               nullptr, nullptr, h->d_verdict, h->d_res, (uint32_t)n, h->stream);
   if (rc != PGX_OK) return rc;
   PGX_HIP(h, pgx::launch_compact(h->d_verdict, (uint32_t)n, 1u, h->d_survivors, count,
                                  h->stream));
-  rc = launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits,
-                          flags & ~PGX_WANT_RESIDUE, rest, h->d_survivors, count, h->d_verdict,
-                          nullptr, 0, h->stream);
-  if (rc != PGX_OK) return rc;
-  PGX_HIP(h, pgx::launch_bump(h->d_verdict, h->d_survivors, count, (uint32_t)n, 1u, h->stream));
+  PGX_HIP(h, pgx::launch_fill(mask, (uint32_t)n, 0u, h->stream));
+  const int S = pgx::kernel_for_bits(max_bits);  // nonzero: the first launch took it
+  pgx::PrpArgs a{};
+  a.cand = h->d_cand;
+  a.ld = (uint32_t)n;
+  a.n = (uint32_t)n;
+  a.buf_limbs = h->limbs;
+  a.mode = mode;
+  a.bases = rest;
+  a.index = h->d_survivors;
+  a.count = count;
+  PGX_HIP(h, pgx::bases_table()[S](a, mask, h->stream));
+  PGX_HIP(h, pgx::launch_settle(h->d_verdict, h->d_survivors, count, (uint32_t)n, mask,
+                                h->stream));
   return PGX_OK;
 }
 
@@ -264,7 +276,7 @@ int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t lim
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_verdict, max_n * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_res, words * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_counters, pgx::kCounters * 4);
-  if (e == hipSuccess && nbases > 1) e = hipMalloc((void**)&h->d_survivors, max_n * 4);
+  if (e == hipSuccess && nbases > 1) e = hipMalloc((void**)&h->d_survivors, 2 * max_n * 4);
   if (e == hipSuccess)
     e = hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);

@@ -29,7 +29,7 @@ struct pgx_runner {
   uint32_t* d_res = nullptr;
   uint32_t* d_counters = nullptr;  // work counters of persistent launches
   // survivors of the first base (runners with several bases): column list of
-  // max_n words, its length in d_counters[0]
+  // max_n words (its length in d_counters[0]), then their per-base pass masks
   uint32_t* d_survivors = nullptr;
   // device construction (pgx_submit_mk): K cache, multipliers, status word
   std::vector<uint32_t> h_K;  // the K held in d_K
@@ -90,9 +90,9 @@ hipError_t launch_build(const uint32_t* K, uint32_t k_words, const uint64_t* m64
                         int32_t sign, uint32_t n, uint32_t* out, uint32_t ld, uint32_t out_words,
                         uint32_t* status, hipStream_t s);
 
-// verdict[index[i]] += add for i < min(*count, n).
-hipError_t launch_bump(uint32_t* verdict, const uint32_t* index, const uint32_t* count, uint32_t n,
-                       uint32_t add, hipStream_t s);
+// verdict[index[i]] = 1 + (trailing one bits of mask[i]) for i < min(*count, n).
+hipError_t launch_settle(uint32_t* verdict, const uint32_t* index, const uint32_t* count,
+                         uint32_t n, const uint32_t* mask, hipStream_t s);
 
 // dst[i] = src[index[i]] for i < n.
 hipError_t launch_gather(const uint32_t* src, const uint32_t* index, uint32_t n, uint32_t* dst,

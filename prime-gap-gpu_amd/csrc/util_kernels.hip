@@ -1,7 +1,7 @@
 // Small device kernels of the runtime: word fill (counter resets, usable as a
 // graph node), verdict compaction (survivor lists for the extra Miller-Rabin
-// bases, computed on the device with one atomic per wavefront), the verdict
-// bump that adds the first base back to the survivors' counts, and gathers.
+// bases, computed on the device with one atomic per wavefront), the verdicts
+// of the survivors from their per-base pass masks, and gathers.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -43,21 +43,23 @@ __global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ src
   if (i < n) dst[i] = src[index[i]];
 }
 
-__global__ __launch_bounds__(256) void k_bump(uint32_t* __restrict__ verdict,
-                                              const uint32_t* __restrict__ index,
-                                              const uint32_t* __restrict__ count, uint32_t n,
-                                              uint32_t add) {
+// Survivor k passed the first base; bit b of mask[k] says whether it passed
+// base b + 1: the verdict counts the leading bases passed.
+__global__ __launch_bounds__(256) void k_settle(uint32_t* __restrict__ verdict,
+                                                const uint32_t* __restrict__ index,
+                                                const uint32_t* __restrict__ count, uint32_t n,
+                                                const uint32_t* __restrict__ mask) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < min(*count, n)) verdict[index[i]] += add;
+  if (i < min(*count, n)) verdict[index[i]] = 1u + (uint32_t)__builtin_ctz(~mask[i]);
 }
 
 }  // namespace
 
-hipError_t launch_bump(uint32_t* verdict, const uint32_t* index, const uint32_t* count, uint32_t n,
-                       uint32_t add, hipStream_t s) {
+hipError_t launch_settle(uint32_t* verdict, const uint32_t* index, const uint32_t* count,
+                         uint32_t n, const uint32_t* mask, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bump, dim3((n + 255) / 256), dim3(256), 0, s, verdict, index, count, n,
-                     add);
+  hipLaunchKernelGGL(k_settle, dim3((n + 255) / 256), dim3(256), 0, s, verdict, index, count, n,
+                     mask);
   return hipGetLastError();
 }
 

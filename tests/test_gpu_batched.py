@@ -137,8 +137,8 @@ PSEUDOPRIMES = [2047, 1373653, 25326001, 3215031751, 2152302898747, 347474966038
 
 
 def two_fills():
-    """Batch size from which pgx_submit compacts the first base's survivors:
-    two wavefronts of 64 on each of the 4 SIMDs of every compute unit."""
+    """Two device fills: two wavefronts of 64 on each of the 4 SIMDs of every
+    compute unit."""
     import torch
     return 2 * 4 * 64 * torch.cuda.get_device_properties(0).multi_processor_count
 
@@ -156,14 +156,13 @@ def one_pass(L, bases, mode):
 
 @pytest.mark.parametrize("bases", [[2, 3, 5, 7, 11, 13, 17], [3, 2, 5], [2, 3]])
 def test_runner_runs_extra_bases_on_survivors(bases):
-    """pgx_submit with several bases and a batch of two device fills or more
-    tests the first base on every candidate and the others on the
-    device-compacted list of those that passed it.  The verdicts equal a single
-    pass over all bases (pgx_test_device) on every column and the oracle's on
-    the pseudoprimes and a sample, with the first base's residues; the list is
-    rebuilt for the next batch, for a batch with no survivors and for
-    candidates built on the device; a batch below two fills takes the single
-    pass and agrees as well."""
+    """pgx_submit with several bases tests the first base on every candidate
+    and the others, one (survivor, base) pair per lane, on the device-compacted
+    list of those that passed it.  The verdicts equal a single pass over all
+    bases (pgx_test_device) on every column and the oracle's on the
+    pseudoprimes and a sample, with the first base's residues; the list and
+    the pass masks are rebuilt for the next batch, a smaller one, a batch with
+    no survivors and candidates built on the device."""
     fills = two_fills()
     P = pgx.to_limbs(PSEUDOPRIMES, 16)
     sieved = gen(337, 2310, 16, 64, fills // 64 + 64)
@@ -178,7 +177,7 @@ def test_runner_runs_extra_bases_on_survivors(bases):
             ov, ores = O.verdicts(L[sample], mode, tuple(bases), residues=True)
             assert np.array_equal(v[sample], ov) and np.array_equal(r[sample], ores)
             assert int((v >= 1).sum()) > int((v == len(bases)).sum())  # later bases fail some
-        say(f"bases {bases}: next batch, below two fills, no survivors")
+        say(f"bases {bases}: next batch, a smaller one, no survivors")
         L2 = gen(337, 2310, 16, 64, fills // 64 + 64, seed_off=1)[:fills + 3000]
         assert np.array_equal(run.run(L2, pgx.MODE_STRONG), one_pass(L2, bases, pgx.MODE_STRONG))
         small = L2[:4096]
