@@ -266,6 +266,28 @@ def bench_host_boundary(cands: np.ndarray, reps: int = 3) -> dict:
         out["pgx_submit_mk_tests_per_s"] = reps * (n // BATCH) * BATCH / (time.perf_counter() - t0)
     out["workload"] = ("headline candidates in 65536-candidate batches from host memory, "
                        "synchronous per batch (one runner, one stream)")
+    # the reference's own structure: three runners, each driven by a host
+    # thread of its own (gpu_testing.cu:203-354 starts one run_gpu_thread per
+    # runner); the library calls release the GIL, so one thread's host copy
+    # overlaps another's kernel
+    from concurrent.futures import ThreadPoolExecutor
+    runners = [pgx.Runner(BATCH, limbs=LIMBS) for _ in range(3)]
+    try:
+        for r in runners:
+            r.run(cands[:BATCH], pgx.MODE_REFERENCE)
+
+        def drive(k):
+            for c0 in range(k * BATCH, n, 3 * BATCH):
+                runners[k].run(cands[c0:c0 + BATCH], pgx.MODE_REFERENCE)
+
+        with ThreadPoolExecutor(3) as ex:
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                list(ex.map(drive, range(3)))
+            out["pgx_run_three_threads_tests_per_s"] = reps * n / (time.perf_counter() - t0)
+    finally:
+        for r in runners:
+            r.close()
     # construction alone (fill_batch, gpu_testing.cu:136-195): the device
     # builder on resident buffers against GMP on one host thread
     import torch
@@ -853,7 +875,8 @@ def main() -> int:
     if "host" in todo:
         extra["host_boundary"] = bench_host_boundary(cands)
         hb = extra["host_boundary"]
-        log(f"host boundary: pgx_run {hb['pgx_run_tests_per_s'] / 1e6:.2f} M/s, pgx_submit_mk "
+        log(f"host boundary: pgx_run {hb['pgx_run_tests_per_s'] / 1e6:.2f} M/s (three runners "
+            f"on three threads {hb['pgx_run_three_threads_tests_per_s'] / 1e6:.2f} M/s), pgx_submit_mk "
             f"{hb['pgx_submit_mk_tests_per_s'] / 1e6:.2f} M/s; construction on the device "
             f"{hb['device_build_candidates_per_s'] / 1e9:.2f} G/s vs GMP one thread "
             f"{hb['cpu_build_candidates_per_s_one_thread'] / 1e6:.2f} M/s "
