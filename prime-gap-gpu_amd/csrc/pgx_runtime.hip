@@ -178,51 +178,63 @@ int launch_plain_bases(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint3
   return PGX_OK;
 }
 
-// The runner's own batch in d_cand (pgx_submit, pgx_submit_mk).  With several
-// bases in a Miller-Rabin mode: the first base on every candidate; compaction
-// of the columns that passed it; every other base on every survivor at once,
-// one (survivor, base) pair per lane, into a pass mask per survivor; then the
-// verdicts of the survivors from their masks.  In one pass over all bases
-// every wavefront holding a probable prime (sieved candidates put a few in
-// nearly every wavefront) runs the bases one after another; here the later
-// bases cost one exponentiation's latency together.  The verdict stays the
-// count of leading bases passed; residues (the first base's) come from the
-// first launch.
+// The runner's own batch in d_cand (pgx_submit, pgx_submit_mk): one pass when
+// the runner has one base (or in FERMAT mode), else the first base and then
+// launch_later_bases.
 int launch_batch(pgx_runner* h, size_t n, uint32_t max_bits, uint32_t flags) {
   const uint32_t mode = flags & PGX_MODE_MASK;
   if (h->bases.count == 1 || mode == PGX_MODE_FERMAT)
     return launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, h->bases,
                               nullptr, nullptr, h->d_verdict, h->d_res, (uint32_t)n, h->stream);
-  pgx::Bases first{}, rest{};
-  first.count = 1;
-  first.a[0] = h->bases.a[0];
-  rest.count = h->bases.count - 1;
-  for (uint32_t k = 1; k < h->bases.count; ++k) rest.a[k - 1] = h->bases.a[k];
-  uint32_t* count = h->d_counters;  // slot 0: persistent launches use slots 1..
-  uint32_t* mask = h->d_survivors + h->max_n;
-  int rc = launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, first,
-                              nullptr, nullptr, h->d_verdict, h->d_res, (uint32_t)n, h->stream);
+  int rc = launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags,
+                              pgx::first_base(h->bases), nullptr, nullptr, h->d_verdict, h->d_res,
+                              (uint32_t)n, h->stream);
   if (rc != PGX_OK) return rc;
-  PGX_HIP(h, pgx::launch_compact(h->d_verdict, (uint32_t)n, 1u, h->d_survivors, count,
-                                 h->stream));
-  PGX_HIP(h, pgx::launch_fill(mask, (uint32_t)n, 0u, h->stream));
-  const int S = pgx::kernel_for_bits(max_bits);  // nonzero: the first launch took it
-  pgx::PrpArgs a{};
-  a.cand = h->d_cand;
-  a.ld = (uint32_t)n;
-  a.n = (uint32_t)n;
-  a.buf_limbs = h->limbs;
-  a.mode = mode;
-  a.bases = rest;
-  a.index = h->d_survivors;
-  a.count = count;
-  PGX_HIP(h, pgx::bases_table()[S](a, mask, h->stream));
-  PGX_HIP(h, pgx::launch_settle(h->d_verdict, h->d_survivors, count, (uint32_t)n, mask,
-                                h->stream));
-  return PGX_OK;
+  return pgx::launch_later_bases(h, (uint32_t)n, max_bits, mode, h->stream);
 }
 
 }  // namespace
+
+pgx::Bases pgx::first_base(const Bases& b) {
+  Bases f{};
+  f.count = 1;
+  f.a[0] = b.a[0];
+  return f;
+}
+
+// Columns 0..n-1 of d_cand (ld = n) hold a batch whose first-base verdicts
+// (0 or 1, or INVALID) are in d_verdict.  The columns that passed are listed
+// on the device (k_compact); every other base of the runner then runs on every
+// survivor at once, one (survivor, base) pair per lane, into a pass mask per
+// survivor (k_prp_bases); k_settle writes each survivor's count of leading
+// bases passed.  In one pass over all bases every wavefront holding a probable
+// prime (sieved candidates put a few in nearly every wavefront) runs the bases
+// one after another; here the later bases cost one exponentiation's latency
+// together.
+int pgx::launch_later_bases(pgx_runner* h, uint32_t n, uint32_t max_bits, uint32_t mode,
+                            hipStream_t stream) {
+  if (h->bases.count < 2 || n == 0) return PGX_OK;
+  const int S = pgx::kernel_for_bits(max_bits);
+  if (S == 0 || !h->d_survivors)
+    return fail(h, PGX_E_ARG, "no survivor pass for %u-bit candidates", max_bits);
+  uint32_t* count = h->d_counters;  // slot 0: persistent launches use slots 1..
+  uint32_t* mask = h->d_survivors + h->max_n;
+  PGX_HIP(h, pgx::launch_compact(h->d_verdict, n, 1u, h->d_survivors, count, stream));
+  PGX_HIP(h, pgx::launch_fill(mask, n, 0u, stream));
+  pgx::PrpArgs a{};
+  a.cand = h->d_cand;
+  a.ld = n;
+  a.n = n;
+  a.buf_limbs = h->limbs;
+  a.mode = mode;
+  a.bases.count = h->bases.count - 1;
+  for (uint32_t k = 1; k < h->bases.count; ++k) a.bases.a[k - 1] = h->bases.a[k];
+  a.index = h->d_survivors;
+  a.count = count;
+  PGX_HIP(h, pgx::bases_table()[S](a, mask, stream));
+  PGX_HIP(h, pgx::launch_settle(h->d_verdict, h->d_survivors, count, n, mask, stream));
+  return PGX_OK;
+}
 
 int pgx::launch_prp(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs, size_t n,
                     uint32_t max_bits, uint32_t flags, uint32_t* d_verdict, uint32_t* d_res,

@@ -78,6 +78,17 @@ int launch_prp(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limb
                uint32_t max_bits, uint32_t flags, uint32_t* d_verdict, uint32_t* d_res,
                uint32_t res_ld, hipStream_t stream);
 
+// The first of the runner's bases alone.
+Bases first_base(const Bases& b);
+
+// After a first-base launch over columns 0..n-1 of the runner's d_cand (ld = n)
+// into d_verdict: the runner's other bases on the columns that passed, one
+// (column, base) pair per lane, and the verdicts of those columns set to the
+// count of leading bases passed.  Uses the runner's survivor buffers and
+// counter slot 0 (one batch at a time per runner).  PGX_E_*.
+int launch_later_bases(pgx_runner* h, uint32_t n, uint32_t max_bits, uint32_t mode,
+                       hipStream_t stream);
+
 // Launch over device buffers with the runner's tuned (or default) shape for
 // this width and mode: persistent one-lane workgroups or lane groups.
 int launch_auto(pgx_runner* h, const PrpArgs& a, uint32_t max_bits, hipStream_t stream);

@@ -718,9 +718,11 @@ int pgx_search_test(pgx_search_t* s, uint32_t* n_found) {
     pa.n = nb;
     pa.buf_limbs = h->limbs;
     pa.mode = PGX_MODE_REFERENCE;
-    pa.bases = h->bases;
+    // several bases: the first here, the others side by side on its survivors
+    pa.bases = pgx::first_base(h->bases);
     pa.verdict = h->d_verdict;
     int rc = pgx::launch_auto(h, pa, s->max_bits, st);
+    if (rc == PGX_OK) rc = pgx::launch_later_bases(h, nb, s->max_bits, PGX_MODE_REFERENCE, st);
     if (rc != PGX_OK) return sfail(s, rc, h->err.c_str());
     hipLaunchKernelGGL(k_consume, dim3((nb + kConsumeBlock - 1) / kConsumeBlock),
                        dim3(kConsumeBlock), 0, st,
