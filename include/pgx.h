@@ -186,7 +186,9 @@ typedef struct {
 } pgx_bucket_t;
 
 typedef struct {
-  uint32_t blocks_per_cu;    /* resident workgroups per compute unit */
+  uint32_t blocks_per_cu;    /* resident workgroups per compute unit; 0 with one lane per
+                                candidate: no persistence, one work item per thread over a
+                                grid that covers the bucket (the next two fields unused) */
   uint32_t waves_per_block;  /* wavefronts per workgroup, 1..4 */
   uint32_t chunk;            /* candidates a wavefront takes per counter bump, multiple of 64 */
   uint32_t lanes_per_number; /* lanes that share one candidate: 1 (persistent workgroups,
@@ -217,7 +219,8 @@ int pgx_compact(pgx_runner_t* h, const uint32_t* d_verdict, uint32_t n, uint32_t
                 uint32_t* d_index, uint32_t* d_count, void* stream);
 
 /* Autotuner: times persistent launch shapes (blocks_per_cu x waves_per_block x
- * chunk) on bucket b with `flags`, keeps the fastest for b's kernel in the
+ * chunk), the plain grid and 2/4/8-lane groups on bucket b with `flags`, keeps
+ * the fastest for b's kernel in the
  * runner (used when tuning is NULL) and returns it in *best with the measured
  * candidates per second in *per_sec (either may be NULL). */
 int pgx_autotune(pgx_runner_t* h, const pgx_bucket_t* b, uint32_t flags, pgx_tuning_t* best,

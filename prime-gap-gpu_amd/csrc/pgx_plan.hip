@@ -26,6 +26,7 @@ pgx_tuning_t default_tuning() {
 }
 
 int check_tuning(pgx_runner* h, const pgx_tuning_t& t) {
+  if (t.blocks_per_cu == 0 && t.lanes_per_number == 1) return PGX_OK;  // plain grid
   if (t.waves_per_block < 1 || t.waves_per_block > 4 || t.blocks_per_cu < 1 ||
       t.blocks_per_cu > 64 || t.chunk < 64 || t.chunk % 64 || t.chunk > 4096)
     return fail(h, PGX_E_ARG, "bad tuning: blocks_per_cu=%u waves_per_block=%u chunk=%u",
@@ -76,11 +77,13 @@ pgx::PersistShape shape_of(pgx_runner* h, const pgx_tuning_t& t) {
   return sh;
 }
 
-// Kernel of a bucket under a tuning: one lane per candidate (persistent), or a
-// group of lanes per candidate (one work item per group, plain grid; the
-// counter is unused).
+// Kernel of a bucket under a tuning: one lane per candidate (persistent, or a
+// plain grid with one work item per thread when blocks_per_cu is 0), or a
+// group of lanes per candidate (one work item per group, plain grid).  The
+// counter serves the persistent form only.
 hipError_t launch_tuned(pgx_runner* h, int S, const pgx_tuning_t& t, const pgx::PrpArgs& a,
                         uint32_t max_bits, uint32_t* counter, hipStream_t s) {
+  if (t.lanes_per_number == 1 && t.blocks_per_cu == 0) return pgx::prp_table()[S](a, s);
   if (t.lanes_per_number == 1) {
     hipError_t e = pgx::launch_fill(counter, 1, 0u, s);
     return e != hipSuccess ? e : pgx::persist_table()[S](a, counter, shape_of(h, t), s);
@@ -227,11 +230,12 @@ int pgx_autotune(pgx_runner_t* h, const pgx_bucket_t* b, uint32_t flags, pgx_tun
           top = t;
         }
       }
-  for (uint32_t lanes : {2u, 4u, 8u}) {  // a group of lanes per candidate
+  for (uint32_t lanes : {1u, 2u, 4u, 8u}) {  // the plain grid, then lane groups
     if (rc != PGX_OK) break;
-    if (!pgx::group_width_for_bits(b->max_bits, (int)lanes)) continue;
+    if (lanes > 1 && !pgx::group_width_for_bits(b->max_bits, (int)lanes)) continue;
     pgx_tuning_t t = default_tuning();
     t.lanes_per_number = lanes;
+    if (lanes == 1) t.blocks_per_cu = 0;
     float best_ms = 0;
     for (int rep = 0; rep < 2 && rc == PGX_OK; ++rep) {
       hipError_t e = hipEventRecord(e0, h->stream);

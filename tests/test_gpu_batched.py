@@ -38,12 +38,14 @@ def test_persistent_equals_plain_and_oracle():
             ov, ores = O.verdicts(L, O.MODE_FERMAT, residues=True)
             assert np.array_equal(a.verdicts(), ov)
             assert np.array_equal(a.residues(), ores)
-            # a non-default shape gives the same answer
-            t = pgx.Tuning(2, 1, 256, 1)
-            b = pgx.DeviceBatch(L)
-            run.test_persistent(b.bucket(), pgx.MODE_FERMAT, residues=True, tuning=t)
-            torch.cuda.synchronize()
-            assert np.array_equal(b.verdicts(), ov)
+            # a non-default shape, and the plain grid (blocks_per_cu 0), give
+            # the same answer
+            for t in (pgx.Tuning(2, 1, 256, 1), pgx.Tuning(0, 0, 0, 1)):
+                b = pgx.DeviceBatch(L)
+                run.test_persistent(b.bucket(), pgx.MODE_FERMAT, residues=True, tuning=t)
+                torch.cuda.synchronize()
+                assert np.array_equal(b.verdicts(), ov)
+                assert np.array_equal(b.residues(), ores)
 
 
 def test_plan_graph_replays_match_eager():
@@ -113,6 +115,15 @@ def test_fermat_then_mr_on_compacted_survivors():
     assert np.array_equal(got[surv], want)
     others = np.setdiff1d(np.arange(n), surv)
     assert np.all(got[others] == np.uint32(-7 & 0xFFFFFFFF))
+    say("the same work list on the plain grid")
+    mr2 = pgx.DeviceBatch(L, residues=False)
+    mr2.verdict.fill_(-7)
+    torch.cuda.synchronize()
+    with pgx.Runner(1, limbs=64, bases=[3, 5]) as extra:
+        extra.test_indexed(mr2.bucket(), index.data_ptr(), count.data_ptr(), pgx.MODE_STRONG,
+                           tuning=pgx.Tuning(0, 0, 0, 1))
+        torch.cuda.synchronize()
+    assert np.array_equal(mr2.verdicts(), got)
 
 
 def test_indexed_with_empty_list():
@@ -216,7 +227,8 @@ def test_bad_tuning_and_bucket_are_rejected():
     L = gen(337, 2310, 16, 1, 64)
     b = pgx.DeviceBatch(L)
     with pgx.Runner(1, limbs=16) as run:
-        for bad in (pgx.Tuning(1, 1, 100, 1), pgx.Tuning(1, 5, 64, 1), pgx.Tuning(1, 1, 64, 3)):
+        for bad in (pgx.Tuning(1, 1, 100, 1), pgx.Tuning(1, 5, 64, 1), pgx.Tuning(1, 1, 64, 3),
+                    pgx.Tuning(0, 4, 64, 2)):  # the plain grid is for one lane per candidate
             with pytest.raises(pgx.PgxError) as e:
                 run.test_persistent(b.bucket(), pgx.MODE_FERMAT, tuning=bad)
             assert e.value.code == pgx.PGX_E_ARG
