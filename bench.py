@@ -731,8 +731,9 @@ def cpu_baseline(cands: np.ndarray, target_cpu_s: float = 15.0) -> dict:
 
 def summary(plain_512: float, frac_512, extra: dict) -> dict:
     """The metric's parts side by side: tests/s at 512/1024/2048 bits (plain
-    launches and the batched/persistent pattern), the INT32 VALU fraction, and
-    the serving p99 latency (None where that leg did not run)."""
+    launches and the batched/persistent pattern), the INT32 VALU fraction, the
+    serving p99 latency, seven Miller-Rabin bases and the host hand-over
+    through pgx_run (None where that leg did not run)."""
     def get(cfg, key):
         return extra.get(cfg, {}).get(key)
     return {
@@ -747,6 +748,9 @@ def summary(plain_512: float, frac_512, extra: dict) -> dict:
         "valu_frac_2048": get("class_2048", "plain_valu_frac"),
         "serving_p50_ms": get("serving", "p50_ms"),
         "serving_p99_ms": get("serving", "p99_ms"),
+        "tests_per_s_512_7_bases_pgx_run": get("rounds", "pgx_run_rounds7_batch65536_tests_per_s"),
+        "tests_per_s_512_from_host_three_threads": get("host_boundary",
+                                                       "pgx_run_three_threads_tests_per_s"),
     }
 
 
