@@ -220,6 +220,9 @@ int pgx_autotune(pgx_runner_t* h, const pgx_bucket_t* b, uint32_t flags, pgx_tun
       for (uint32_t ch : chunks) {
         if (rc != PGX_OK) break;
         if (w * bl > 32) continue;  // beyond the 32 wavefronts a CU can hold
+        // more wavefronts than chunks of work: some would idle, and where the
+        // busy ones land decides the time (the plain grid covers this case)
+        if ((uint64_t)h->n_cu * bl * w > ((uint64_t)sample.n + ch - 1) / ch) continue;
         pgx_tuning_t t{bl, w, ch, 1};
         float shape_ms = 0;
         for (int rep = 0; rep < kTuneReps && rc == PGX_OK; ++rep) {
@@ -254,7 +257,7 @@ int pgx_autotune(pgx_runner_t* h, const pgx_bucket_t* b, uint32_t flags, pgx_tun
       if (e != hipSuccess) rc = fail(h, PGX_E_HIP, "autotune timing: %s", hipGetErrorString(e));
       if (rc == PGX_OK && ms > shape_ms) shape_ms = ms;  // the slowest rep
     }
-    if (rc == PGX_OK && shape_ms < top_ms) {
+    if (rc == PGX_OK && (top_ms == 0 || shape_ms < top_ms)) {  // 0: no persistent shape fit
       top_ms = shape_ms;
       top = t;
     }
