@@ -9,10 +9,12 @@ launches of 65536 candidates each (base-2 Fermat, residues written), inputs
 already resident in HBM in limb-interleaved layout.
 
 Printed JSON line: tests/s (whole job), the dominant kernel's roofline against
-the measured v_mad_u64_u32 issue rate (integer-VALU bound; no HBM or MFMA
-bound applies: ~130 B of HBM traffic per test against ~2e5 multiply-adds), and
-the GMP CPU baseline (mpz_powm(2, N-1, N)) timed on a bounded sample of the
-same candidates on the host cores.
+MI355X's full-rate VALU issue (256 CUs x 64 lanes x 2.4 GHz = 3.93e13
+v_mad_u64_u32 per second; integer-VALU bound, no HBM or MFMA bound applies:
+~130 B of HBM traffic per test against ~2e5 multiply-adds), beside the rate a
+pure-mad probe reaches on the same device, and the GMP CPU baseline
+(mpz_powm(2, N-1, N)) timed on a bounded sample of the same candidates on the
+host cores.
 """
 from __future__ import annotations
 
@@ -39,6 +41,11 @@ PER_M = 16384
 P, D, LIMBS = 337, 2310, 16
 X_MAX = 450000
 SEED = 0x5EED0002
+# MI355X full-rate VALU issue: 256 CUs x 4 SIMDs x 16 lanes per clock at the
+# 2.4 GHz peak engine clock, one v_mad_u64_u32 (32x32->64 multiply-add) per
+# lane per issue.  AMD publishes no INT32 VALU figure; this matches the
+# published fp64 vector FMA rate (78.6 TFLOP/s = 2 x 3.93e13).
+PEAK_SPEC = 256 * 64 * 2.4e9
 
 
 def log(msg: str) -> None:
@@ -86,13 +93,25 @@ def time_ms(stream, fn, reps: int) -> float:
     return e0.elapsed_time(e1) / reps
 
 
+def host_threads() -> tuple[int, dict]:
+    """Threads for the CPU legs: every core this process may run on
+    (sched_getaffinity), capped by OMP_NUM_THREADS where the environment sets
+    it (the CPU share a shared host grants one GPU's jobs); and the counts
+    behind that choice."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    cap = int(omp) if omp.isdigit() and int(omp) > 0 else aff
+    return min(aff, cap), {"affinity_cores": aff, "cpu_count": os.cpu_count(),
+                           "omp_num_threads": int(omp) if omp.isdigit() else None}
+
+
 def cpu_rate(cands: np.ndarray, cpu_seconds: float = 3.0) -> dict:
-    """GMP mpz_powm(2, N-1, N) tests/s on min(16, affinity) host threads over a
-    bounded sample of about cpu_seconds of CPU work (the oracle, timed as the
-    reported CPU baseline)."""
+    """GMP mpz_powm(2, N-1, N) tests/s on host_threads() threads over a bounded
+    sample of about cpu_seconds of CPU work (the oracle, timed as the reported
+    CPU baseline)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads, _ = host_threads()
     sec1, _ = oracle.time_powm2(cands[:32], threads=1, warmup=True)
     per = sec1 / min(32, len(cands))
     n = int(min(len(cands), max(256, cpu_seconds / per)))
@@ -313,6 +332,77 @@ def bench_host_boundary(cands: np.ndarray, reps: int = 3) -> dict:
     out["cpu_build_candidates_per_s_one_thread"] = BATCH / cpu_s
     out["build_equals_gmp"] = bool(np.array_equal(host, want) and int(d_st.cpu()[0]) == 0)
     return out
+
+
+def bench_dropin(n_x: int = 8, passes: int = 3) -> dict:
+    """The drop-in gpu_testing.cpp run by tests/dropin/harness under the
+    reference's own dispatch protocol: GPU_BATCHES = 3 host threads, each with
+    a runner and a GPUBatch of up to 32768 multipliers (candidates built on the
+    device), fill_batch / process_finished_batch on the host, the X barrier
+    between steps.  Interval: P=337, d=2310, m in [1e9, 1e9+1e7), composites
+    from the device sieve by the primes below 1e8, first n_x coprime X.  PCIe
+    and host work included.  Pass 1 warms up (runner creation); the rate is
+    over the later passes.  Cross-checked X by X against pgx_search_step over
+    the same interval."""
+    import struct
+    import subprocess
+    import tempfile
+
+    import pgx
+    harness = os.path.join(ROOT, "tests", "dropin", "harness")
+    if not os.path.exists(harness):
+        return {"skipped": "tests/dropin/harness is not built"}
+    P, d, m_start, m_inc, mp = 337, 2310, 10**9, 10_000_000, 100_000_000
+    K = pgx.primorial_over(P, d)
+    # the reference's X list (SieveData ctor, gap_search_gpu.cpp:236-254): even
+    # X < 10 P with no prime factor among the primes <= P that do not divide d
+    kp = [q for q in range(3, P + 1) if all(q % r for r in range(2, int(q ** 0.5) + 1))
+          and d % q]
+    xs = [x for x in range(2, 10 * P, 2) if all(x % q for q in kp)][:n_x]
+    with pgx.Runner(1 << 20, limbs=16) as run:
+        with pgx.Search(run, K, P, d, m_start, m_inc, mp) as s:
+            active = s.active()
+            comps = []
+            for X in xs:
+                s.sieve(X)
+                comps.append(s.bitmap(pgx.SEARCH_COMPOSITES))
+        with pgx.Search(run, K, P, d, m_start, m_inc, mp) as s:
+            want = [s.step(X) for X in xs]
+            want_found = s.bitmap(pgx.SEARCH_FOUND)
+    with tempfile.TemporaryDirectory() as tmp:
+        inp, out = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+        with open(inp, "wb") as f:
+            f.write(b"PGXH")
+            f.write(struct.pack("<IIQIIII", P, d, m_start, m_inc, len(xs), len(comps[0]),
+                                len(active)))
+            f.write(active.astype("<u4").tobytes())
+            for X, c in zip(xs, comps):
+                f.write(struct.pack("<I", X))
+                f.write(np.ascontiguousarray(c, dtype="<u4").tobytes())
+        r = subprocess.run([harness, inp, out, str(passes)], capture_output=True, text=True,
+                           errors="replace", timeout=600)
+        if r.returncode != 0:
+            return {"failed": (r.stdout + r.stderr)[-2000:]}
+        raw = open(out, "rb").read()
+    (nx,) = struct.unpack_from("<I", raw, 0)
+    pairs = np.frombuffer(raw, dtype="<u4", count=2 * nx, offset=4).reshape(nx, 2)
+    (w,) = struct.unpack_from("<I", raw, 4 + 8 * nx)
+    found = np.frombuffer(raw, dtype="<u4", count=w, offset=8 + 8 * nx)
+    secs = [float(v) for v in r.stdout.split("seconds per pass:")[1].split()]
+    tests = int(pairs[:, 0].sum())
+    equal = bool([tuple(int(v) for v in p) for p in pairs] == [tuple(x) for x in want]
+                 and np.array_equal(found, want_found[:w]) and not want_found[w:].any())
+    timed = secs[1:] if len(secs) > 1 else secs
+    return {
+        "workload": f"drop-in gpu_testing.cpp under the reference's dispatch (3 GPU threads, "
+                    f"batches of 32768 built on the device): P={P} d={d} m in [1e9, 1e9+1e7), "
+                    f"primes < 1e8, first {n_x} coprime X",
+        "tests_per_pass": tests, "found_per_pass": int(pairs[:, 1].sum()),
+        "seconds_per_pass": secs,
+        "tests_per_s": tests / float(np.mean(timed)),
+        "equal_to_device_search": equal,
+        "harness_line": r.stdout.strip().splitlines()[-1],
+    }
 
 
 def bench_rounds(cands: np.ndarray, stream, n: int = 1 << 18) -> dict:
@@ -710,7 +800,7 @@ def cpu_baseline(cands: np.ndarray, target_cpu_s: float = 15.0) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the GMP path, timed as the reported CPU baseline
 
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads, counts = host_threads()
     calib = cands[:512]
     sec1, _ = oracle.time_powm2(calib, threads=1, warmup=True)
     per_test = sec1 / len(calib)
@@ -723,9 +813,12 @@ def cpu_baseline(cands: np.ndarray, target_cpu_s: float = 15.0) -> dict:
         "cores": threads,
         "kind": "port",
         "sample": f"first {n} of the same candidates, GMP mpz_powm(2, N-1, N), {threads} "
-                  f"threads static shards, {sec:.2f} s wall ({per_test * 1e6:.1f} us/test "
+                  f"threads static shards ({counts['affinity_cores']} cores in this process's "
+                  f"affinity, {counts['cpu_count']} on the host, OMP_NUM_THREADS="
+                  f"{counts['omp_num_threads']}), {sec:.2f} s wall ({per_test * 1e6:.1f} us/test "
                   f"single-thread); {fermat} Fermat PRPs",
         "single_thread_tests_per_s": 1.0 / per_test,
+        "host_cores": counts,
     }
 
 
@@ -751,6 +844,7 @@ def summary(plain_512: float, frac_512, extra: dict) -> dict:
         "tests_per_s_512_7_bases_pgx_run": get("rounds", "pgx_run_rounds7_batch65536_tests_per_s"),
         "tests_per_s_512_from_host_three_threads": get("host_boundary",
                                                        "pgx_run_three_threads_tests_per_s"),
+        "tests_per_s_512_dropin_three_gpu_threads": get("dropin", "tests_per_s"),
     }
 
 
@@ -762,7 +856,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline plain-kernel workload")
-    ap.add_argument("--extras", default="persistent,streams,host,rounds,1024,2048,mixed,large,search,serving",
+    ap.add_argument("--extras", default="persistent,streams,host,dropin,rounds,1024,2048,mixed,"
+                                        "large,search,serving",
                     help="comma-separated extra configurations to run")
     ap.add_argument("--serve-group", default="1000,32768,4,88",
                     help="bits,items,lanes[,cu%%[,slots]]: serving iterations of classes at "
@@ -864,8 +959,12 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
     rate = {name: pgx.valu_rate(op) for op, name in pgx.VALU_OPS.items()}
-    peak = rate["v_mad_u64_u32"]
-    log("VALU rates (lane-ops/s): " + ", ".join(f"{k}={v:.3e}" for k, v in rate.items() if v))
+    probe = rate["v_mad_u64_u32"]
+    # every valu_frac below is against the full-rate issue of the spec; the
+    # probe's own rate (what a pure-mad kernel reaches here) is reported beside
+    peak = PEAK_SPEC
+    log("VALU rates (lane-ops/s): " + ", ".join(f"{k}={v:.3e}" for k, v in rate.items() if v)
+        + f"; spec {PEAK_SPEC:.3e}")
 
     extra = {}
     todo = set() if args.no_extras else set(args.extras.split(","))
@@ -885,6 +984,12 @@ def main() -> int:
             f"{hb['device_build_candidates_per_s'] / 1e9:.2f} G/s vs GMP one thread "
             f"{hb['cpu_build_candidates_per_s_one_thread'] / 1e6:.2f} M/s "
             f"(equal: {hb['build_equals_gmp']})")
+    if "dropin" in todo:
+        extra["dropin"] = bench_dropin()
+        dr = extra["dropin"]
+        log(f"drop-in: {dr.get('tests_per_s', 0) / 1e6:.2f} M tests/s through gpu_testing.cpp "
+            f"on 3 GPU threads (equal to the device search: {dr.get('equal_to_device_search')})"
+            if "tests_per_s" in dr else f"drop-in: {dr}")
     if "rounds" in todo:
         extra["rounds"] = bench_rounds(cands, stream)
         r = extra["rounds"]
@@ -959,11 +1064,17 @@ def main() -> int:
         "roofline": {
             "bound": "valu-int32",
             "achieved": achieved / 1e12,
-            "peak": peak / 1e12 if peak else None,
+            "peak": peak / 1e12,
             "unit": "TMAC/s",
-            "frac": achieved / peak if peak else None,
+            "frac": achieved / peak,
+            "peak_source": "256 CU x 64 lanes x 2.4 GHz full-rate v_mad_u64_u32 issue",
+            "peak_probe": probe / 1e12 if probe else None,
+            "frac_probe": achieved / probe if probe else None,
             "traffic": traffic,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)",
+            "traffic_source": ("constant read from the rocprofv3 --pmc passes of this kernel in "
+                               "profiles/k_prp_plain_s17_p337_d2310_2e20.md, not measured in "
+                               "this run") if traffic else None,
             "algorithmic_bytes_per_launch": bytes_launch,
             "kernel": kname,
             "macs_per_launch": macs_launch,
@@ -976,7 +1087,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "parity_sample_ok": parity,
-        "summary": summary(tests_per_s, achieved / peak if peak else None, extra),
+        "summary": summary(tests_per_s, achieved / peak, extra),
         "configs": extra,
     }
     print(json.dumps(line), flush=True)

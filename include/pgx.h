@@ -162,8 +162,8 @@ int pgx_search_found(pgx_search_t* s, uint32_t* m_i, uint32_t* x, size_t cap, si
 int pgx_search_bitmap(pgx_search_t* s, uint32_t which, uint32_t* out, size_t cap, size_t* n);
 const char* pgx_search_last_error(pgx_search_t* s);
 
-/* Message of the last failing call on h (or of the last failing create when
- * h is NULL).  Never NULL. */
+/* Message of the last failing call on h (or, when h is NULL, of the last
+ * failing pgx_runner_create on the calling thread).  Never NULL. */
 const char* pgx_last_error(const pgx_runner_t* h);
 
 /* ---- batched persistent launches ----------------------------------------

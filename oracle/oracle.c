@@ -183,7 +183,23 @@ static uint64_t xoshiro_next(xoshiro_t* g) {
   return r;
 }
 
-/* first outputs of the stream, to pin the PRNG against published vectors */
+/* The generator's parts, each checkable against published known answers:
+ * splitmix64 outputs from a seed, and xoshiro256** outputs from a given state
+ * (the authors' reference algorithms, prng.di.unimi.it). */
+int orc_splitmix_stream(uint64_t seed, uint32_t count, uint64_t* out) {
+  for (uint32_t i = 0; i < count; ++i) out[i] = splitmix(&seed);
+  return ORC_OK;
+}
+
+int orc_xoshiro_from_state(const uint64_t* state, uint32_t count, uint64_t* out) {
+  xoshiro_t g;
+  for (int k = 0; k < 4; ++k) g.s[k] = state[k];
+  for (uint32_t i = 0; i < count; ++i) out[i] = xoshiro_next(&g);
+  return ORC_OK;
+}
+
+/* first outputs of the generator's stream: xoshiro256** seeded with four
+ * splitmix64 outputs of `seed` */
 int orc_xoshiro_stream(uint64_t seed, uint32_t count, uint64_t* out) {
   xoshiro_t g;
   for (int k = 0; k < 4; ++k) g.s[k] = splitmix(&seed);

@@ -38,6 +38,8 @@ def _load():
     lib.orc_probab_prime.argtypes = [u32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_int32)]
     lib.orc_xoshiro_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint32, u64p]
+    lib.orc_splitmix_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint32, u64p]
+    lib.orc_xoshiro_from_state.argtypes = [u64p, ctypes.c_uint32, u64p]
     lib.orc_gen_multipliers.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                         ctypes.c_uint32, ctypes.c_uint32, u64p]
     lib.orc_gen_sieved.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u64p, ctypes.c_uint32,
@@ -102,6 +104,20 @@ def probab_prime(limbs: np.ndarray, reps: int = 25) -> np.ndarray:
     out = np.zeros(a.shape[0], dtype=np.int32)
     lib.orc_probab_prime(_u32(a), a.shape[0], a.shape[1], reps,
                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return out
+
+
+def splitmix_stream(seed: int, count: int) -> np.ndarray:
+    out = np.zeros(count, dtype=np.uint64)
+    lib.orc_splitmix_stream(ctypes.c_uint64(seed), count, _u64(out))
+    return out
+
+
+def xoshiro_from_state(state, count: int) -> np.ndarray:
+    st = np.ascontiguousarray(state, dtype=np.uint64)
+    assert st.shape == (4,)
+    out = np.zeros(count, dtype=np.uint64)
+    lib.orc_xoshiro_from_state(_u64(st), count, _u64(out))
     return out
 
 

@@ -18,6 +18,11 @@ struct PrpArgs {
   uint32_t ld;
   uint32_t n;          // work items (upper bound when count is set)
   uint32_t buf_limbs;  // rows stored in cand (rows beyond the kernel's width must be zero)
+  // Rows [kernel width, check_limbs) are read and a column with a nonzero word
+  // there gets PGX_VERDICT_INVALID: buf_limbs where max_bits is the caller's
+  // promise (device-resident inputs), 0 where the library derived max_bits
+  // from the candidates themselves (no extra reads).
+  uint32_t check_limbs;
   uint32_t mode;       // PGX_MODE_*
   Bases bases;
   uint32_t* verdict;         // indexed by column

@@ -61,6 +61,7 @@ pgx::PrpArgs args_of(pgx_runner* h, const pgx_bucket_t* b, uint32_t flags) {
   a.ld = b->ld;
   a.n = b->n;
   a.buf_limbs = b->limbs;
+  a.check_limbs = b->limbs;  // max_bits is the caller's bound
   a.mode = flags & PGX_MODE_MASK;
   a.bases = h->bases;
   a.verdict = b->d_verdict;
@@ -100,12 +101,8 @@ pgx_tuning_t tuning_for(pgx_runner* h, int S, uint32_t flags, const pgx_tuning_t
 }
 
 // Next counter slot of the runner: launches in flight on different streams
-// (up to kCounters - 1 of them) never share a counter.
-uint32_t* next_counter(pgx_runner* h) {
-  static thread_local uint32_t slot = 0;
-  slot = (slot + 1) % (pgx::kCounters - 1);
-  return h->d_counters + 1 + slot;
-}
+// (up to kCounters - 1 of them, from any host threads) never share a counter.
+uint32_t* next_counter(pgx_runner* h) { return h->d_counters + h->slots.next(); }
 
 int launch_persistent(pgx_runner* h, const pgx_bucket_t* b, const pgx_tuning_t* tuning,
                       uint32_t flags, const uint32_t* d_index, const uint32_t* d_count,

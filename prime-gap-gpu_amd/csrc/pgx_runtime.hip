@@ -37,8 +37,10 @@ PrpLaunchFn* group_table(int G) {
   return G == 2 ? two : G == 4 ? four : eight;
 }
 
+// Per thread: runners are created from several host threads at once (one per
+// GPU thread of the reference's dispatch).
 std::string& create_error() {
-  static std::string msg = "no error";
+  static thread_local std::string msg = "no error";
   return msg;
 }
 
@@ -152,9 +154,12 @@ void release(pgx_runner* h) {
 namespace {
 
 // Plain launch with an explicit base list, over columns 0..n-1 or, with index
-// and count, over the work list index[0..min(*count, n)).
+// and count, over the work list index[0..min(*count, n)).  `checked`: max_bits
+// was derived from the candidates by the library, so the kernel skips the
+// check of the rows above its width.
 int launch_plain_bases(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs,
-                       size_t n, uint32_t max_bits, uint32_t flags, const pgx::Bases& bases,
+                       size_t n, uint32_t max_bits, uint32_t flags, bool checked,
+                       const pgx::Bases& bases,
                        const uint32_t* index, const uint32_t* count, uint32_t* d_verdict,
                        uint32_t* d_res, uint32_t res_ld, hipStream_t stream) {
   const int S = pgx::kernel_for_bits(max_bits);
@@ -167,6 +172,7 @@ int launch_plain_bases(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint3
   a.ld = ld;
   a.n = (uint32_t)n;
   a.buf_limbs = limbs;
+  a.check_limbs = checked ? 0u : limbs;
   a.mode = flags & PGX_MODE_MASK;
   a.bases = bases;
   a.index = index;
@@ -184,9 +190,10 @@ int launch_plain_bases(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint3
 int launch_batch(pgx_runner* h, size_t n, uint32_t max_bits, uint32_t flags) {
   const uint32_t mode = flags & PGX_MODE_MASK;
   if (h->bases.count == 1 || mode == PGX_MODE_FERMAT)
-    return launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, h->bases,
-                              nullptr, nullptr, h->d_verdict, h->d_res, (uint32_t)n, h->stream);
-  int rc = launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags,
+    return launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, true,
+                              h->bases, nullptr, nullptr, h->d_verdict, h->d_res, (uint32_t)n,
+                              h->stream);
+  int rc = launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, true,
                               pgx::first_base(h->bases), nullptr, nullptr, h->d_verdict, h->d_res,
                               (uint32_t)n, h->stream);
   if (rc != PGX_OK) return rc;
@@ -239,8 +246,8 @@ int pgx::launch_later_bases(pgx_runner* h, uint32_t n, uint32_t max_bits, uint32
 int pgx::launch_prp(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limbs, size_t n,
                     uint32_t max_bits, uint32_t flags, uint32_t* d_verdict, uint32_t* d_res,
                     uint32_t res_ld, hipStream_t stream) {
-  return launch_plain_bases(h, d_cand, ld, limbs, n, max_bits, flags, h->bases, nullptr, nullptr,
-                            d_verdict, d_res, res_ld, stream);
+  return launch_plain_bases(h, d_cand, ld, limbs, n, max_bits, flags, false, h->bases, nullptr,
+                            nullptr, d_verdict, d_res, res_ld, stream);
 }
 
 using pgx::launch_prp;

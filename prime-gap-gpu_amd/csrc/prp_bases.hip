@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kBlock) void k_prp_bases(PrpArgs a, uint32_t* mask)
 #pragma unroll
   for (int j = 0; j < W; ++j) w[j] = (j < (int)a.buf_limbs) ? col[j * ld] : 0u;
   uint32_t spill = 0;  // words above the kernel's width must be zero
-  for (uint32_t j = W; j < a.buf_limbs; ++j) spill |= col[j * ld];
+  for (uint32_t j = W; j < a.check_limbs; ++j) spill |= col[j * ld];
   const int L = bit_length_words<W>(w);
   if (spill != 0 || !(w[0] & 1u) || L < 2 || L > Radix<S>::MAX_BITS) return;
   const int t = ctz_nm1_words<W>(w);

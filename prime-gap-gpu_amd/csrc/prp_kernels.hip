@@ -38,7 +38,7 @@ __device__ __forceinline__ void test_column(const PrpArgs& a, uint32_t p) {
 #pragma unroll
   for (int j = 0; j < W; ++j) w[j] = (j < (int)a.buf_limbs) ? c[j * ld] : 0u;
   uint32_t spill = 0;  // words above the kernel's width must be zero
-  for (uint32_t j = W; j < a.buf_limbs; ++j) spill |= c[j * ld];
+  for (uint32_t j = W; j < a.check_limbs; ++j) spill |= c[j * ld];
   uint32_t rw[W];
   uint32_t v = PGX_VERDICT_INVALID;
   if (spill == 0) {

@@ -11,6 +11,11 @@
 
 #include "../../include/pgx.h"
 #include "launch.hpp"
+#include "slots.hpp"
+
+namespace pgx {
+constexpr uint32_t kCounters = 64;  // counter slots per runner
+}  // namespace pgx
 
 struct pgx_runner {
   int device = 0;
@@ -28,6 +33,7 @@ struct pgx_runner {
   uint32_t* d_verdict = nullptr;
   uint32_t* d_res = nullptr;
   uint32_t* d_counters = nullptr;  // work counters of persistent launches
+  pgx::CounterSlots slots{pgx::kCounters};  // the slot each persistent launch takes
   // survivors of the first base (runners with several bases): column list of
   // max_n words (its length in d_counters[0]), then their per-base pass masks
   uint32_t* d_survivors = nullptr;
@@ -50,8 +56,7 @@ struct pgx_runner {
 
 namespace pgx {
 
-constexpr uint32_t kCounters = 64;  // counter slots per runner
-
+// Message of the last failing runner create on the calling thread.
 std::string& create_error();
 
 inline int fail(pgx_runner* h, int code, const char* fmt, ...) {

@@ -6,8 +6,10 @@
 // unchanged.
 //
 // What changes for MI355X:
-//  * GPU_BATCH_SIZE is 65536 (reference 8192, gpu_testing.cu:57-58): one lane
-//    owns one candidate, and 65536 lanes are one wavefront per SIMD on 256 CUs.
+//  * GPU_BATCH_SIZE is 32768 (reference 8192, gpu_testing.cu:57-58), the
+//    largest size prime_gap_test accepts (gap_search_gpu.cpp:1076-1077): one
+//    lane owns one candidate, so the GPU_BATCHES = 3 batches in flight put
+//    1.5 wavefronts on each of the 1024 SIMDs of 256 CUs.
 //  * fill_batch only lists the multipliers m = m_start + m_i; the candidates
 //    N = m * K + x are built on the device (pgx_submit_mk), replacing
 //    mpz_mul_ui + mpz_add_ui per candidate (gpu_testing.cu:171-172) and the
@@ -50,7 +52,7 @@ static const uint32_t kBits = 1024;
 static const uint32_t kLimbs = kBits / 32;
 
 const size_t GPU_BATCHES = 3;
-const size_t GPU_BATCH_SIZE = 64 * 1024;
+const size_t GPU_BATCH_SIZE = 32 * 1024;
 static const uint32_t kRounds = 1;  // base-2 strong test, as the reference's ROUNDS
 
 namespace {
@@ -112,6 +114,8 @@ static void fill_batch(TestData& test_data, GPUBatch& batch, std::vector<uint64_
 void run_gpu_thread(int runner_num, int verbose, TestData& test_data, GPUBatch& batch,
                     const mpz_t& K_in) {
   pgx_runner_t* runner = nullptr;
+  bool holding = false;  // this thread holds batch's lock
+  bool running = false;  // ... and counts in test_data.running_batches
   try {
     pthread_setname_np(pthread_self(), ("GPU(" + std::to_string(runner_num) + ")").c_str());
     const std::vector<uint32_t> K = limbs_of(K_in);
@@ -123,7 +127,9 @@ void run_gpu_thread(int runner_num, int verbose, TestData& test_data, GPUBatch& 
     size_t processed = 0;
     while (is_running && stop_queue <= 1) {
       batch.wait_for_state_and_lock(GPUBatch::EMPTY);
+      holding = true;
       if (!is_running || stop_queue > 1) {
+        holding = false;
         batch.unlock();
         break;
       }
@@ -135,6 +141,7 @@ void run_gpu_thread(int runner_num, int verbose, TestData& test_data, GPUBatch& 
       if (batch.i == 0) {  // nothing left at this X: this batch retires
         assert(test_data.test_i == test_data.unknown_m_i.size());
         batch.state = GPUBatch::WAITING;
+        holding = false;
         batch.unlock();
         if (--test_data.active_batches == 0) {
           assert(test_data.running_batches == 0);
@@ -145,6 +152,7 @@ void run_gpu_thread(int runner_num, int verbose, TestData& test_data, GPUBatch& 
         continue;
       }
       test_data.running_batches += 1;
+      running = true;
       batch.state = GPUBatch::RUNNING;
       test_data.unlock();
 
@@ -175,7 +183,9 @@ void run_gpu_thread(int runner_num, int verbose, TestData& test_data, GPUBatch& 
                batch.i, batch.x, batch.primes_in_batch);
 
       batch.state = GPUBatch::EMPTY;
+      holding = false;
       batch.unlock();
+      running = false;
       test_data.running_batches -= 1;
     }
     if (verbose >= 2) {
@@ -187,13 +197,29 @@ void run_gpu_thread(int runner_num, int verbose, TestData& test_data, GPUBatch& 
       test_data.state.notify_all();
     }
   } catch (const std::exception& e) {
+    // The reference exits inside CUDA_CHECK (miller_rabin.h:57-68).  Here a
+    // failing call stops the search instead: release this thread's batch and
+    // move test_data to DONE, so that run_testing_thread, blocked in
+    // wait_for_state_and_lock(DONE) (gap_search_gpu.cpp:953-954), wakes, sees
+    // is_running == false and winds down.  (An atomic wait only returns once
+    // the value changes, so the state itself must change, not just a notify.)
     std::cout << "ERROR in run_gpu_thread\n" << e.what() << std::endl;
     is_running = false;
+    if (running) test_data.running_batches -= 1;
+    if (holding) {
+      batch.state = GPUBatch::DONE;
+      batch.unlock();
+    }
+    test_data.state = TestData::DONE;
+    test_data.state.notify_all();
   }
   if (runner) pgx_runner_destroy(runner);
 }
 
 void gpu_state_and_checks(const mpz_t& K_in, const uint64_t m_end) {
+  // prime_gap_test asserts the same set (gap_search_gpu.cpp:1076-1077).
+  static_assert(GPU_BATCH_SIZE == 1024 || GPU_BATCH_SIZE == 2048 || GPU_BATCH_SIZE == 4096 ||
+                GPU_BATCH_SIZE == 8192 || GPU_BATCH_SIZE == 16384 || GPU_BATCH_SIZE == 32768);
   const size_t n_bits = mpz_sizeinbase(K_in, 2) + (size_t)std::log2((double)m_end) + 4;
   printf("TESTING PRIMES ON GPU (gfx950, one lane per candidate)\n");
   printf("BITS=%u (host limb buffer)\n", kBits);

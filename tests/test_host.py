@@ -4,13 +4,14 @@ host build of the kernels' per-lane routine vs the oracle."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
 
 import oracle as O
 import pgx
-from conftest import ROOT, limbs_needed
+from conftest import PKG, ROOT, limbs_needed
 
 
 def test_library_exports_every_declared_symbol():
@@ -230,3 +231,38 @@ def test_lane_routine_base_two_alone_in_every_mode(emu):
             assert np.array_equal(np.minimum(v23, 1), v), mode
             _, ores = O.verdicts(arr, O.MODE_FERMAT, residues=True)
             assert np.array_equal(r, ores), mode
+
+
+def test_counter_slots_two_threads_share_one_runner(tmp_path):
+    """Persistent launches take work-counter slots from their runner
+    (csrc/slots.hpp).  Two host threads launching on one runner must get
+    different slots: the first kCounters - 1 = 63 slots handed out, whichever
+    thread took them, are all distinct and lie in 1..63 (slot 0 is the survivor
+    pass's)."""
+    src = tmp_path / "slots.cpp"
+    src.write_text(
+        '#include "slots.hpp"\n'
+        "#include <cstdio>\n#include <thread>\n#include <vector>\n#include <atomic>\n"
+        "int main() {\n"
+        "  pgx::CounterSlots slots(64);\n"
+        "  std::vector<unsigned> a(31), b(32);\n"
+        "  std::atomic<int> go{0};\n"
+        "  auto run = [&](std::vector<unsigned>& out) {\n"
+        "    go.fetch_add(1); while (go.load() < 2) {}\n"
+        "    for (auto& v : out) v = slots.next(); };\n"
+        "  std::thread t1(run, std::ref(a)), t2(run, std::ref(b));\n"
+        "  t1.join(); t2.join();\n"
+        "  for (unsigned v : a) printf(\"%u \", v);\n"
+        "  for (unsigned v : b) printf(\"%u \", v);\n"
+        "  for (int k = 0; k < 126; ++k) printf(\"%u \", slots.next());\n"
+        "  return 0; }\n")
+    exe = tmp_path / "slots"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", f"-I{os.path.join(PKG, 'csrc')}",
+                    str(src), "-o", str(exe)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                           check=True).stdout.split()]
+    first = vals[:63]
+    assert sorted(first) == list(range(1, 64))
+    # afterwards the slots keep coming round in order, each once per 63
+    rest = vals[63:]
+    assert all(rest[i] % 63 + 1 == rest[i + 1] for i in range(len(rest) - 1))

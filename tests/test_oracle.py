@@ -100,6 +100,24 @@ def test_python_restatement_matches_oracle():
         assert bool(v & 4) == O.py_ref_verdict(n, 2)
 
 
+def test_prng_parts_match_published_known_answers():
+    """The generator's two parts against the known answers published with
+    their reference algorithms: splitmix64 from seed 0, and xoshiro256** from
+    the state {1, 2, 3, 4} (the vectors other implementations test against);
+    the seeded stream is xoshiro256** on four splitmix64 outputs."""
+    sm = O.splitmix_stream(0, 4)
+    assert [int(v) for v in sm] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4,
+                                    0x06C45D188009454F, 0xF88BB8A8724C81EC]
+    xo = O.xoshiro_from_state([1, 2, 3, 4], 10)
+    assert [int(v) for v in xo] == [11520, 0, 1509978240, 1215971899390074240,
+                                    1216172134540287360, 607988272756665600,
+                                    16172922978634559625, 8476171486693032832,
+                                    10595114339597558777, 2904607092377533576]
+    for seed in (0, 0x5EED0001, 2**64 - 1):
+        assert np.array_equal(O.xoshiro_stream(seed, 16),
+                              O.xoshiro_from_state(O.splitmix_stream(seed, 4), 16))
+
+
 def test_prng_matches_fixture(vectors):
     p = vectors["prng"]
     got = O.xoshiro_stream(p["seed"], len(p["first"]))
