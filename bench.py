@@ -488,6 +488,69 @@ def bench_1024(stream, peak: float, reps: int) -> dict:
     }
 
 
+# Width classes timed with the plain kernel alone (--plain-classes): (P, d,
+# buffer words, multipliers, survivors per multiplier), the same generators and
+# seeds as the 1024-bit, 1260-bit (Fermat -> MR) and 2048-bit legs.
+PLAIN_CLASSES = {
+    "1024": (701, 210, 32, 16, 16384),    # 2^18 candidates, ~987 bits, k_prp_plain<36>
+    "1260": (907, 30030, 64, 32, 8192),   # 2^18 candidates, ~1260 bits, k_prp_plain<46>
+    "2048": (1447, 210, 64, 16, 4096),    # 2^16 candidates, ~2020 bits, k_prp_plain<74>
+}
+
+
+def bench_plain_class(name: str, stream, peak: float, reps: int) -> dict:
+    """One width class through the plain kernel alone: launches of 65536
+    candidates (base-2 Fermat + residues, HBM-resident), HIP events around
+    every launch on the launch stream, the roofline of that kernel, a parity
+    sample against GMP."""
+    import torch
+
+    import pgx
+    P, d, words, n_m, per_m = PLAIN_CLASSES[name]
+    L = gen(P, d, words, n_m, per_m, SEED + P)
+    a = pgx.DeviceBatch(L)
+    run = pgx.Runner(1, limbs=words)
+    starts = list(range(0, a.n, BATCH))
+
+    def plain(events=None):
+        for k, c0 in enumerate(starts):
+            if events is not None:
+                events[k][0].record(stream)
+            run.test_device(a.cand.data_ptr() + 4 * c0, a.ld, a.limbs, min(BATCH, a.n - c0),
+                            a.max_bits, pgx.MODE_FERMAT, a.verdict.data_ptr() + 4 * c0,
+                            a.res.data_ptr() + 4 * c0, a.ld, stream.cuda_stream)
+            if events is not None:
+                events[k][1].record(stream)
+    plain()
+    torch.cuda.synchronize()
+    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in starts]
+          for _ in range(reps)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for r in range(reps):
+        plain(ev[r])
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    launch_ms = float(np.mean([x[0].elapsed_time(x[1]) for er in ev for x in er]))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    ov, ores = oracle.verdicts(L[:256], oracle.MODE_FERMAT, residues=True)
+    ok = bool(np.array_equal(a.verdicts()[:256], ov) and np.array_equal(a.residues()[:256], ores))
+    run.close()
+    bits = median_bits(L)
+    per_launch = min(BATCH, a.n)
+    macs = work_per_test(bits) * per_launch
+    return {
+        "workload": f"P={P} d={d}, {a.n} sieved candidates (~{bits} bits), {words}-word buffers, "
+                    f"plain launches of {per_launch}, base-2 Fermat + residues",
+        "bits": bits, "kernel": f"k_prp_plain<{kernel_limbs(a.max_bits)},true>",
+        "tests_per_s": a.n / (ms * 1e-3), "avg_launch_ms": launch_ms,
+        "macs_per_launch": macs, "valu_frac": macs / (launch_ms * 1e-3) / peak,
+        "parity_sample_ok": ok,
+    }
+
+
 def bench_2048(stream, peak: float, reps: int) -> dict:
     """The 2048-bit class: P=1447, d=210 (~2020-bit N in 64-word buffers),
     2^16 sieved candidates, Fermat + residues: the plain one-lane launch
@@ -839,6 +902,9 @@ def summary(plain_512: float, frac_512, extra: dict) -> dict:
         "valu_frac_512": frac_512,
         "valu_frac_1024": get("class_1024", "plain_valu_frac"),
         "valu_frac_2048": get("class_2048", "plain_valu_frac"),
+        "valu_frac_1024_plain_class": get("plain_1024", "valu_frac"),
+        "valu_frac_1260_plain_class": get("plain_1260", "valu_frac"),
+        "valu_frac_2048_plain_class": get("plain_2048", "valu_frac"),
         "serving_p50_ms": get("serving", "p50_ms"),
         "serving_p99_ms": get("serving", "p99_ms"),
         "tests_per_s_512_7_bases_pgx_run": get("rounds", "pgx_run_rounds7_batch65536_tests_per_s"),
@@ -865,6 +931,9 @@ def main() -> int:
                          "candidate (0,0,4 = never); cu%% > 0 runs those classes on that share of "
                          "the CUs and the narrower classes on the rest; `slots` iterations of a "
                          "class may be in flight at once")
+    ap.add_argument("--plain-classes", default="",
+                    help="comma-separated width classes (1024,1260,2048) to time through the "
+                         "plain kernel alone, one entry per class in configs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -968,6 +1037,11 @@ def main() -> int:
 
     extra = {}
     todo = set() if args.no_extras else set(args.extras.split(","))
+    for name in [c for c in args.plain_classes.split(",") if c]:
+        extra[f"plain_{name}"] = bench_plain_class(name, stream, peak, max(2, args.steps // 4))
+        pc = extra[f"plain_{name}"]
+        log(f"plain {name}-bit class: {pc['tests_per_s'] / 1e6:.3f} M tests/s, launch "
+            f"{pc['avg_launch_ms']:.3f} ms, VALU frac {pc['valu_frac']:.3f} ({pc['kernel']})")
     if "persistent" in todo:
         extra["same_workload_persistent"] = bench_persistent(cands, stream, peak, L,
                                                              max(2, args.steps // 4))

@@ -13,10 +13,13 @@
 //     the atomic, and a state wait that returns holding the lock once the state
 //     is the one asked for or the search stopped;
 //   * the X loop of run_testing_thread (:846-1047), with the sieve thread's
-//     hand-off replaced by the file: per X, finalize (:711-735: the active m_i
-//     not marked composite), try_set_testing_data (:355-395: swap the list in,
-//     drop m_i already found, ACTIVE, every batch EMPTY), wait for DONE, merge
-//     the batches' stats, remove_vector(active, found) (:340-351), next X.
+//     hand-off replaced by the file: the sieve thread finalizes each X's list
+//     ahead of the tests (:711-735: the active m_i not marked composite, from
+//     an active list that may predate the last X's primes, OPEN_SIEVES = 4
+//     ahead), so the lists are made before the timed passes; per X the loop
+//     does try_set_testing_data (:355-395: swap the list in, drop m_i already
+//     found, ACTIVE, every batch EMPTY), waits for DONE, merges the batches'
+//     stats, remove_vector(active, found) (:340-351), next X.
 //
 // Input (little-endian): "PGXH", P, d, m_start (u64), m_inc, n_x, words,
 // n_active, active[n_active], then per X: X, composites[words] (bit t <->
@@ -179,23 +182,31 @@ struct PassResult {
   double seconds = 0;
 };
 
+// finalize (gap_search_gpu.cpp:711-735): the tests list of every X from the
+// interval's active list, as the sieve thread hands them over ahead of time.
+std::vector<std::vector<uint32_t>> finalize_all(const Input& in) {
+  std::vector<std::vector<uint32_t>> lists(in.xs.size());
+  for (size_t k = 0; k < in.xs.size(); ++k)
+    for (uint32_t m_i : in.active)
+      if (!bit(in.composites[k], m_i)) lists[k].push_back(m_i);
+  return lists;
+}
+
 // One pass of run_testing_thread's X loop over every X of the input.
-PassResult run_pass(const Input& in, TestData& test_data, std::deque<GPUBatch>& batches) {
+PassResult run_pass(const Input& in, std::vector<std::vector<uint32_t>> lists,
+                    TestData& test_data, std::deque<GPUBatch>& batches) {
   PassResult r;
   std::vector<uint32_t> active = in.active;
   test_data.full_reset();
   const auto t0 = high_resolution_clock::now();
   for (size_t k = 0; k < in.xs.size() && is_running; ++k) {
     const uint32_t X = in.xs[k];
-    // finalize: the tests list of X (gap_search_gpu.cpp:711-735)
-    std::vector<uint32_t> tests;
-    for (uint32_t m_i : active)
-      if (!bit(in.composites[k], m_i)) tests.push_back(m_i);
+    std::vector<uint32_t>& tests = lists[k];
     // try_set_testing_data (:355-395)
     test_data.lock();
     assert(test_data.state == TestData::WAITING);
     assert(test_data.unknown_m_i.empty() && test_data.running_batches == 0);
-    const std::vector<uint32_t> found_before = test_data.found_prime_m_i;
+    const uint64_t primes_before = test_data.gpu_stats.total_primes;
     test_data.m_start = in.m_start;
     test_data.testing_x = X;
     test_data.unknown_m_i.swap(tests);
@@ -231,10 +242,7 @@ PassResult run_pass(const Input& in, TestData& test_data, std::deque<GPUBatch>& 
       batch.stats.reset();
       batch.unlock();
     }
-    uint32_t newly = 0;
-    for (size_t w = 0; w < found_before.size(); ++w)
-      newly += __builtin_popcount(test_data.found_prime_m_i[w] & ~found_before[w]);
-    r.found.push_back(newly);
+    r.found.push_back((uint32_t)(test_data.gpu_stats.total_primes - primes_before));
     remove_found(active, test_data.found_prime_m_i);  // :977
     test_data.test_i = 0;
     test_data.unknown_m_i.clear();
@@ -286,10 +294,11 @@ int main(int argc, char** argv) {
     threads.emplace_back(run_gpu_thread, (int)i, verbose, std::ref(test_data),
                          std::ref(batches[i]), std::ref(K));
 
+  const std::vector<std::vector<uint32_t>> lists = finalize_all(in);
   PassResult last;
   std::vector<double> secs;
   for (int p = 0; p < passes && is_running; ++p) {
-    last = run_pass(in, test_data, batches);
+    last = run_pass(in, lists, test_data, batches);  // the lists are copied per pass
     secs.push_back(last.seconds);
   }
   const bool ok = is_running;
