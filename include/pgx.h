@@ -87,6 +87,25 @@ int pgx_test_device(pgx_runner_t* h, const uint32_t* d_cand, uint32_t ld, uint32
                     size_t n, uint32_t max_bits, uint32_t flags, uint32_t* d_verdict,
                     uint32_t* d_residues, uint32_t res_ld, void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  pgx_submit's batch on device-resident candidates without a
+ * runner: the first of `bases` (NULL: the first nbases primes) on every
+ * column, then, with several bases outside FERMAT mode, the columns that
+ * passed it compacted on the device and the other bases side by side on
+ * them.  Verdicts (and residues of the first base with PGX_WANT_RESIDUE) are
+ * those of pgx_test_device / pgx_run.  Layout and max_bits as pgx_test_device;
+ * launches on `stream` (NULL = the device's null stream) and returns without
+ * waiting.  Allocates nothing: its scratch (survivor list, pass masks,
+ * count; 8 MiB) is a device array of the library, one per GPU, held until the
+ * process ends; n is limited to PGX_NOALLOC_MAX_N (PGX_E_CAPACITY above).
+ * Calls that overlap on one GPU share that scratch and race: serialize them
+ * (one stream, or events).  Messages: pgx_last_error(NULL) on this thread. */
+int pgx_test_device_noalloc(int device, const uint32_t* d_cand, uint32_t ld, uint32_t limbs,
+                            size_t n, uint32_t max_bits, uint32_t flags, const uint32_t* bases,
+                            uint32_t nbases, uint32_t* d_verdict, uint32_t* d_residues,
+                            uint32_t res_ld, void* stream);
+
 /* fill_batch on the device (gpu_testing.cu:136-195 builds N = m*K + x with
  * mpz_mul_ui + mpz_add_ui per candidate, then run_test exports the limbs).
  * pgx_submit_mk uploads K (k_words little-endian words; cached while the same

@@ -30,5 +30,6 @@
 #define PGX_MAX_BASES 16
 #define PGX_MAX_BASE 255 /* largest Miller-Rabin base (the first 16 primes end at 53) */
 #define PGX_MAX_BITS 2068 /* widest candidate the kernels take (74 radix-2^28 limbs) */
+#define PGX_NOALLOC_MAX_N (1u << 20) /* most candidates per pgx_test_device_noalloc call */
 
 #endif /* PGX_DEFS_H */

@@ -48,10 +48,8 @@ std::string& create_error() {
 
 using pgx::fail;
 
-namespace {
-
 // First `count` primes, the reference's generate_primes (miller_rabin.h:272-289).
-void first_primes(uint32_t count, uint32_t* out) {
+void pgx::first_primes(uint32_t count, uint32_t* out) {
   uint32_t found = 0;
   for (uint32_t c = 2; found < count; ++c) {
     bool prime = true;
@@ -63,6 +61,8 @@ void first_primes(uint32_t count, uint32_t* out) {
     if (prime) out[found++] = c;
   }
 }
+
+namespace {
 
 uint32_t bit_length(const uint32_t* x, uint32_t limbs) {
   for (int j = (int)limbs - 1; j >= 0; --j)
@@ -218,28 +218,43 @@ pgx::Bases pgx::first_base(const Bases& b) {
 // prime (sieved candidates put a few in nearly every wavefront) runs the bases
 // one after another; here the later bases cost one exponentiation's latency
 // together.
+hipError_t pgx::launch_survivor_bases(const uint32_t* cand, uint32_t ld, uint32_t limbs,
+                                      uint32_t check_limbs, uint32_t n, int S, uint32_t mode,
+                                      const Bases& later, uint32_t* verdict, uint32_t* index,
+                                      uint32_t* count, uint32_t* mask, hipStream_t stream) {
+  if (n == 0 || later.count == 0) return hipSuccess;
+  hipError_t e = pgx::launch_compact(verdict, n, 1u, index, count, stream);
+  if (e == hipSuccess) e = pgx::launch_fill(mask, n, 0u, stream);
+  if (e != hipSuccess) return e;
+  pgx::PrpArgs a{};
+  a.cand = cand;
+  a.ld = ld;
+  a.n = n;
+  a.buf_limbs = limbs;
+  a.check_limbs = check_limbs;
+  a.mode = mode;
+  a.bases = later;
+  a.index = index;
+  a.count = count;
+  e = pgx::bases_table()[S](a, mask, stream);
+  if (e != hipSuccess) return e;
+  return pgx::launch_settle(verdict, index, count, n, mask, stream);
+}
+
 int pgx::launch_later_bases(pgx_runner* h, uint32_t n, uint32_t max_bits, uint32_t mode,
                             hipStream_t stream) {
   if (h->bases.count < 2 || n == 0) return PGX_OK;
   const int S = pgx::kernel_for_bits(max_bits);
   if (S == 0 || !h->d_survivors)
     return fail(h, PGX_E_ARG, "no survivor pass for %u-bit candidates", max_bits);
-  uint32_t* count = h->d_counters;  // slot 0: persistent launches use slots 1..
-  uint32_t* mask = h->d_survivors + h->max_n;
-  PGX_HIP(h, pgx::launch_compact(h->d_verdict, n, 1u, h->d_survivors, count, stream));
-  PGX_HIP(h, pgx::launch_fill(mask, n, 0u, stream));
-  pgx::PrpArgs a{};
-  a.cand = h->d_cand;
-  a.ld = n;
-  a.n = n;
-  a.buf_limbs = h->limbs;
-  a.mode = mode;
-  a.bases.count = h->bases.count - 1;
-  for (uint32_t k = 1; k < h->bases.count; ++k) a.bases.a[k - 1] = h->bases.a[k];
-  a.index = h->d_survivors;
-  a.count = count;
-  PGX_HIP(h, pgx::bases_table()[S](a, mask, stream));
-  PGX_HIP(h, pgx::launch_settle(h->d_verdict, h->d_survivors, count, n, mask, stream));
+  pgx::Bases later{};
+  later.count = h->bases.count - 1;
+  for (uint32_t k = 1; k < h->bases.count; ++k) later.a[k - 1] = h->bases.a[k];
+  // count in counter slot 0 (persistent launches take slots 1..), the
+  // survivor list and the masks in d_survivors
+  PGX_HIP(h, pgx::launch_survivor_bases(h->d_cand, n, h->limbs, 0u, n, S, mode, later,
+                                        h->d_verdict, h->d_survivors, h->d_counters,
+                                        h->d_survivors + h->max_n, stream));
   return PGX_OK;
 }
 
@@ -284,7 +299,7 @@ int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t lim
       h->bases.a[k] = bases[k];
     }
   } else {
-    first_primes(nbases, h->bases.a);
+    pgx::first_primes(nbases, h->bases.a);
   }
   const size_t words = (size_t)limbs * max_n;
   hipError_t e = hipSetDevice(device);

@@ -59,6 +59,9 @@ namespace pgx {
 // Message of the last failing runner create on the calling thread.
 std::string& create_error();
 
+// First `count` primes (the reference's generate_primes, miller_rabin.h:272-289).
+void first_primes(uint32_t count, uint32_t* out);
+
 inline int fail(pgx_runner* h, int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -85,6 +88,18 @@ int launch_prp(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint32_t limb
 
 // The first of the runner's bases alone.
 Bases first_base(const Bases& b);
+
+// The later bases of a several-base batch over the survivors of its first base
+// (verdicts 1 in `verdict`, columns 0..n-1 of cand): k_compact lists them in
+// index (its length in *count), k_prp_bases tests every other base on every
+// survivor at once into mask (n words, cleared here), k_settle writes each
+// survivor's count of leading bases passed.  index, count and mask are the
+// caller's scratch; every kernel is a plain launch (no memset), so the
+// sequence may be captured into a graph.
+hipError_t launch_survivor_bases(const uint32_t* cand, uint32_t ld, uint32_t limbs,
+                                 uint32_t check_limbs, uint32_t n, int S, uint32_t mode,
+                                 const Bases& later, uint32_t* verdict, uint32_t* index,
+                                 uint32_t* count, uint32_t* mask, hipStream_t stream);
 
 // After a first-base launch over columns 0..n-1 of the runner's d_cand (ld = n)
 // into d_verdict: the runner's other bases on the columns that passed, one

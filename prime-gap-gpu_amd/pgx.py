@@ -48,6 +48,7 @@ MODE_REFERENCE = 0
 MODE_STRONG = 1
 MODE_FERMAT = 2
 WANT_RESIDUE = 4
+NOALLOC_MAX_N = 1 << 20  # PGX_NOALLOC_MAX_N
 
 FLAG_FERMAT = 1
 FLAG_SPRP = 2
@@ -171,6 +172,9 @@ def _load() -> ctypes.CDLL:
     lib.pgx_run.argtypes = [vp, u32p, ctypes.c_size_t, ctypes.c_uint32, u32p, u32p]
     lib.pgx_test_device.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_size_t,
                                     ctypes.c_uint32, ctypes.c_uint32, vp, vp, ctypes.c_uint32, vp]
+    lib.pgx_test_device_noalloc.argtypes = [ctypes.c_int, vp, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
+                                            u32p, ctypes.c_uint32, vp, vp, ctypes.c_uint32, vp]
     lib.pgx_last_error.argtypes = [vp]
     lib.pgx_last_error.restype = ctypes.c_char_p
     lib.pgx_gen_sieved.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u64p, ctypes.c_uint32,
@@ -336,6 +340,21 @@ def valu_rate(op: int, device: int = 0, blocks: int = 8192, iters: int = 4096) -
     if rc != PGX_OK:
         raise PgxError(rc, f"pgx_valu_rate({op}) failed")
     return out.value
+
+
+def test_device_noalloc(cand_ptr: int, ld: int, limbs: int, n: int, max_bits: int, mode: int,
+                        bases, verdict_ptr: int, residue_ptr: int = 0, res_ld: int = 0,
+                        stream: int = 0, device: int = 0) -> None:
+    """pgx_test_device_noalloc: the several-base batch on device buffers (raw
+    pointers) with no runner and no allocation."""
+    b = np.asarray(list(bases), dtype=np.uint32)
+    flags = mode | (WANT_RESIDUE if residue_ptr else 0)
+    rc = lib.pgx_test_device_noalloc(device, ctypes.c_void_p(cand_ptr), ld, limbs, n, max_bits,
+                                     flags, _u32(b), len(b), ctypes.c_void_p(verdict_ptr),
+                                     ctypes.c_void_p(residue_ptr or None), res_ld,
+                                     ctypes.c_void_p(stream or None))
+    if rc != PGX_OK:
+        raise PgxError(rc, lib.pgx_last_error(None).decode())
 
 
 # ---- runner ---------------------------------------------------------------
