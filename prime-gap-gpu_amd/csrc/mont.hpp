@@ -62,6 +62,24 @@ PGX_FN bool wave_any(bool pred) {
 #endif
 }
 
+// Largest v < 2^bits over the active lanes of the wavefront, as a
+// wavefront-uniform value (bisection over ballots: exact, and the result lives
+// in scalar registers); a host build emulates one lane and returns v.
+template <int kBits>
+PGX_FN uint32_t wave_max(uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t m = 0;
+#pragma unroll
+  for (int b = kBits - 1; b >= 0; --b) {
+    const uint32_t c = m | (1u << b);
+    if (__ballot(v >= c) != 0) m = c;
+  }
+  return m;
+#else
+  return v;
+#endif
+}
+
 // -N^-1 mod 2^32 for odd n0 (Newton iteration, 3 -> 6 -> 12 -> 24 -> 48 bits);
 // its low RB bits are -N^-1 mod 2^RB.
 PGX_FN uint32_t neg_inv32(uint32_t n0) {

@@ -18,7 +18,13 @@
 //
 // One loop covers all bits L-2 .. 0 of N-1: the bits below t are zero, so the
 // strong-test checks of y_0..y_{t-1} ride on the same squaring call site and
-// only one copy of the unrolled squaring is emitted per base kind.
+// only one copy of the unrolled squaring is emitted per base kind.  The bit
+// index is the same on every lane of a wavefront (it starts below the
+// wavefront's longest N; a shorter N starts from Mont(1), whose squarings
+// leave it Mont(1) until its own top bit multiplies in the base), so the loop
+// control, the exponent word loads and the guard of the strong-test checks are
+// scalar: a wavefront issues one instruction at a time, and per-lane control
+// in the loop cost as many issue slots as the squaring's carries.
 #pragma once
 
 #include "../../include/pgx_defs.h"
@@ -74,12 +80,22 @@ template <int S, class WordFn, class Step>
 PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&one)[S], const uint32_t (&n)[S],
                              uint32_t ninv, int L, int t, WordFn word, const Step& step,
                              uint32_t (&res)[S]) {
-  bool sprp = false;
+  // a word, not a bool: a bool set in a divergent block is kept as a lane
+  // mask in scalar registers, merged on every pass of the loop
+  uint32_t sprp = 0;
   if (t == L - 1) {  // N - 1 = 2^t: y_0 = a
     canon<S>(x, n);
-    sprp = equal<S>(x, one) || is_neg<S>(x, one, n);
+    sprp = (equal<S>(x, one) || is_neg<S>(x, one, n)) ? 1u : 0u;
   }
-  uint32_t w = 0;
+  // wavefront-uniform bounds: the longest N and the largest t of the wavefront
+  const int Lw = (int)wave_max<12>((uint32_t)L);
+  const int tw = (int)wave_max<12>((uint32_t)t);
+  if (L != Lw) {  // bits Lw-2 .. L of N-1 are zero: start from Mont(1)
+#pragma unroll
+    for (int j = 0; j < S; ++j) x[j] = one[j];
+  }
+  const int top = (Lw - 2) >> 5;  // word of the first bit
+  uint32_t w_next = word(top);
 #if defined(__HIP_DEVICE_COMPILE__)
   // Place the exponent loop just after a 256-byte boundary (the padding runs
   // once).  With the loop body identical instruction for instruction, its
@@ -90,20 +106,28 @@ PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&one)[S], const u
   // 11, 17, 20 and 46, -5% at 36; so the radix-2^29 kernels (S <= 31) align.
   if constexpr (S <= 31) asm volatile(".p2align 8\n\ts_nop 0\n\ts_nop 0");
 #endif
-  for (int p = L - 2; p >= 0; --p) {
-    if (p == L - 2 || (p & 31) == 31) w = word(p >> 5);
-    // bit p of N-1: bit p of N for p >= 1, zero for p == 0
-    const uint32_t bit = (p > 0) ? ((w >> (p & 31)) & 1u) : 0u;
-    if constexpr (Step::kFolded) {
-      mont_sqr_dbl<S, Step::kShiftOperands>(x, n, ninv, bit);
-    } else {
-      mont_sqr<S>(x, n, ninv);
-      step(x, n, ninv, bit);
-    }
-    if (p <= t && p >= 1) {  // x = Mont(y_{t-p})
-      canon<S>(x, n);
-      sprp = sprp || is_neg<S>(x, one, n) || (p == t && equal<S>(x, one));
-    }
+  for (int wi = top; wi >= 0; --wi) {
+    // 32 bits of N-1 (bit 0 of N-1 is zero; the other bits are N's); the next
+    // word is requested one block of 32 squarings ahead
+    const uint32_t w = (wi == 0) ? (w_next & ~1u) : w_next;
+    if (wi > 0) w_next = word(wi - 1);
+    int b = (wi == top) ? ((Lw - 2) & 31) : 31;
+    do {
+      const uint32_t bit = (w >> b) & 1u;
+      if constexpr (Step::kFolded) {
+        mont_sqr_dbl<S, Step::kShiftOperands>(x, n, ninv, bit);
+      } else {
+        mont_sqr<S>(x, n, ninv);
+        step(x, n, ninv, bit);
+      }
+      const int p = 32 * wi + b;
+      if (p <= tw) {                // uniform: some lane checks at this bit
+        if (p <= t && p >= 1) {     // x = Mont(y_{t-p})
+          canon<S>(x, n);
+          if (is_neg<S>(x, one, n) || (p == t && equal<S>(x, one))) sprp = 1u;
+        }
+      }
+    } while (--b >= 0);
   }
 #pragma unroll
   for (int j = 0; j < S; ++j) res[j] = x[j];
