@@ -554,7 +554,8 @@ def bench_plain_class(name: str, stream, peak: float, reps: int) -> dict:
 def bench_2048(stream, peak: float, reps: int) -> dict:
     """The 2048-bit class: P=1447, d=210 (~2020-bit N in 64-word buffers),
     2^16 sieved candidates, Fermat + residues: the plain one-lane launch
-    against groups of 4 and 8 lanes per candidate, and the autotuner's pick."""
+    against groups of 4, 8, 16 and 32 lanes per candidate, and the
+    autotuner's pick."""
     import pgx
     L = gen(1447, 210, 64, 16, 4096, SEED + 1447)
     a = pgx.DeviceBatch(L)
@@ -568,9 +569,10 @@ def bench_2048(stream, peak: float, reps: int) -> dict:
         run.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True,
                             tuning=pgx.Tuning(8, 4, 64, 4), stream=stream.cuda_stream)
 
-    def lanes8():
-        run.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True,
-                            tuning=pgx.Tuning(8, 4, 64, 8), stream=stream.cuda_stream)
+    def lanes(g):
+        return lambda: run.test_persistent(a.bucket(), pgx.MODE_FERMAT, residues=True,
+                                           tuning=pgx.Tuning(8, 4, 64, g),
+                                           stream=stream.cuda_stream)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     ov, ores = oracle.verdicts(L[:256], oracle.MODE_FERMAT, residues=True)
@@ -578,7 +580,8 @@ def bench_2048(stream, peak: float, reps: int) -> dict:
                        "Fermat + residues", "bits": median_bits(L)}
     w = work_per_test(out["bits"]) * a.n
     ok = True
-    for name, fn in (("plain", plain), ("lanes4", lanes4), ("lanes8", lanes8)):
+    for name, fn in (("plain", plain), ("lanes4", lanes4), ("lanes8", lanes(8)),
+                     ("lanes16", lanes(16)), ("lanes32", lanes(32))):
         fn()
         ms = time_ms(stream, fn, reps)
         ok = ok and bool(np.array_equal(a.verdicts()[:256], ov) and
@@ -644,7 +647,7 @@ def bench_large(stream, peak: float, reps: int) -> dict:
     pipeline()
     ms_fermat = time_ms(stream, fermat_only, reps)
     by_lanes = {}
-    for lanes in (1, 2, 4, 8):  # lanes per candidate of the Miller-Rabin stage
+    for lanes in (1, 2, 4, 8, 16, 32):  # lanes per candidate of the Miller-Rabin stage
         mr_lanes[0] = lanes
         pipeline()
         by_lanes[lanes] = time_ms(stream, pipeline, reps)
@@ -821,6 +824,7 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
         wall = (time.monotonic_ns() - start) * 1e-9
         gc.enable()
         tests = sum(r[2] for r in results.values())
+        st = srv.stats()  # warm-up iterations included
     lat = np.array(lat) * 1e-6
     # CPU leg: GMP tests of each side in ascending X until its first PRP, one
     # thread, over a sample of request templates
@@ -833,7 +837,7 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
             oracle.verdicts(tmpl[k][side][:upto], oracle.MODE_REFERENCE, (2,))
             cpu_tests += upto
     cpu_req_s = len(check) / (time.perf_counter() - t0)
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads, _ = host_threads()
     consistent = all(results[r][:2] == want[pick[r]] for r in results if pick[r] in want)
     by_t = {}
     for r, v in results.items():
@@ -856,7 +860,39 @@ def bench_serving(rate: float = 2e4, duration_s: float = 1.0, templates_per_p: i
         "gpu_requests_per_s": n_req / wall,
         "group_min_bits": group_min_bits, "group_max_items": group_max_items,
         "group_lanes": group_lanes,
+        # device time per iteration and the device's idle time between an
+        # iteration and the next on the same stream (host scheduling turnaround,
+        # or no work): the part a device-side loop could take away
+        "iterations": st["iterations"],
+        "iteration_ms_mean": st["iter_ms"] / max(1, st["iterations"]),
+        "gap_ms_mean": st["gap_ms"] / max(1, st["gaps"]),
+        "gap_fraction": st["gap_ms"] / max(1e-9, st["gap_ms"] + st["iter_ms"]),
     }
+
+
+def bench_serving_capacity(cfg: dict, rates=(2e4, 2.5e4, 3e4, 3.5e4, 4e4, 6e4, 8e4),
+                           p99_limit_ms: float = 20.0) -> dict:
+    """Offered load from 2e4 requests/s upward, each load for up to 1 s and at
+    most 40,000 requests: served rate, p50 / p99 and tests/s per load.  The
+    capacity is the highest offered load whose p99 stays within the limit; the
+    sweep stops after the first load over it.  Arrivals are generated by this
+    Python loop, so at the top loads its own lateness counts as latency."""
+    points = []
+    for r in rates:
+        s = bench_serving(rate=r, duration_s=min(1.0, 40000 / r), **cfg)
+        points.append({"offered_per_s": r, "served_per_s": s["gpu_requests_per_s"],
+                       "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"],
+                       "tests_per_s": s["tests_per_s"],
+                       "results_ok": s["results_match_oracle_and_repeat"],
+                       "iteration_ms_mean": s["iteration_ms_mean"],
+                       "gap_ms_mean": s["gap_ms_mean"], "gap_fraction": s["gap_fraction"]})
+        log(f"serving at {r:.0f}/s offered: served {s['gpu_requests_per_s']:.0f}/s, "
+            f"p50 {s['p50_ms']:.2f} ms, p99 {s['p99_ms']:.2f} ms")
+        if s["p99_ms"] > p99_limit_ms:
+            break
+    ok = [p["offered_per_s"] for p in points if p["p99_ms"] <= p99_limit_ms]
+    return {"p99_limit_ms": p99_limit_ms, "points": points,
+            "max_offered_per_s_within_limit": max(ok) if ok else None}
 
 
 def cpu_baseline(cands: np.ndarray, target_cpu_s: float = 15.0) -> dict:
@@ -911,6 +947,10 @@ def summary(plain_512: float, frac_512, extra: dict) -> dict:
         "tests_per_s_512_from_host_three_threads": get("host_boundary",
                                                        "pgx_run_three_threads_tests_per_s"),
         "tests_per_s_512_dropin_three_gpu_threads": get("dropin", "tests_per_s"),
+        "tests_per_s_2048_lanes16": get("class_2048", "lanes16_tests_per_s"),
+        "tests_per_s_2048_lanes32": get("class_2048", "lanes32_tests_per_s"),
+        "serving_capacity_per_s_p99_within_20ms": (extra.get("serving", {}).get("capacity") or {})
+        .get("max_offered_per_s_within_limit"),
     }
 
 
@@ -925,7 +965,7 @@ def main() -> int:
     ap.add_argument("--extras", default="persistent,streams,host,dropin,rounds,1024,2048,mixed,"
                                         "large,search,serving",
                     help="comma-separated extra configurations to run")
-    ap.add_argument("--serve-group", default="1000,32768,4,88",
+    ap.add_argument("--serve-group", default="1000,32768,16,88",
                     help="bits,items,lanes[,cu%%[,slots]]: serving iterations of classes at "
                          "least `bits` wide with at most `items` candidates use `lanes` lanes per "
                          "candidate (0,0,4 = never); cu%% > 0 runs those classes on that share of "
@@ -1077,9 +1117,9 @@ def main() -> int:
     if "2048" in todo:
         extra["class_2048"] = bench_2048(stream, peak, max(2, args.steps // 4))
         c = extra["class_2048"]
-        log(f"2048-bit: plain {c['plain_tests_per_s'] / 1e6:.3f} M/s, 4 lanes "
-            f"{c['lanes4_tests_per_s'] / 1e6:.3f} M/s, 8 lanes {c['lanes8_tests_per_s'] / 1e6:.3f} "
-            f"M/s, tuned {c['tuned_tests_per_s'] / 1e6:.3f} M/s "
+        log(f"2048-bit: plain {c['plain_tests_per_s'] / 1e6:.3f} M/s, 4/8/16/32 lanes "
+            + "/".join(f"{c[f'lanes{g}_tests_per_s'] / 1e6:.3f}" for g in (4, 8, 16, 32))
+            + f" M/s, tuned {c['tuned_tests_per_s'] / 1e6:.3f} M/s "
             f"{c['tuned']}, cpu {c['cpu']['tests_per_s']:.0f}/s")
     if "mixed" in todo:
         extra["mixed_graph"] = bench_mixed(stream, peak, max(2, args.steps // 4))
@@ -1104,6 +1144,19 @@ def main() -> int:
             k: one[k] for k in ("p50_ms", "p99_ms", "p50_p99_ms_by_P", "tests_per_s",
                                 "results_match_oracle_and_repeat")}
         srv["plain_512_tests_per_s"] = tests_per_s
+        # the same stream with other group sizes for the wide class
+        srv["by_group_lanes"] = {}
+        for g in sorted({4, 8, 16, 32} - {gl}):
+            o = bench_serving(group_min_bits=gb, group_max_items=gi, group_lanes=g,
+                              wide_cu_percent=wide, slots_per_class=slots)
+            srv["by_group_lanes"][str(g)] = {
+                k: o[k] for k in ("p50_ms", "p99_ms", "p50_p99_ms_by_P", "tests_per_s",
+                                  "results_match_oracle_and_repeat")}
+            log(f"serving, {g}-lane groups for the wide class: p50 {o['p50_ms']:.2f} ms "
+                f"p99 {o['p99_ms']:.2f} ms, by P {o['p50_p99_ms_by_P']}")
+        srv["capacity"] = bench_serving_capacity(dict(
+            group_min_bits=gb, group_max_items=gi, group_lanes=gl, wide_cu_percent=wide,
+            slots_per_class=slots))
         extra["serving"] = srv
         log(f"serving: p50 {srv['p50_ms']:.2f} ms p99 {srv['p99_ms']:.2f} ms, "
             f"{srv['tests_per_s'] / 1e6:.2f} M tests/s, by P {srv['p50_p99_ms_by_P']}")

@@ -211,9 +211,10 @@ typedef struct {
   uint32_t waves_per_block;  /* wavefronts per workgroup, 1..4 */
   uint32_t chunk;            /* candidates a wavefront takes per counter bump, multiple of 64 */
   uint32_t lanes_per_number; /* lanes that share one candidate: 1 (persistent workgroups,
-                                the fields above apply), 2, 4 or 8 (a lane group per
-                                candidate, one work item per group: for launches with few
-                                candidates) */
+                                the fields above apply), 2, 4, 8, 16 or 32 (a lane group
+                                per candidate, one work item per group: for launches with
+                                few candidates; the reference spreads a candidate over 4 or
+                                8 threads, gpu_testing.cu:51) */
 } pgx_tuning_t;
 
 typedef struct pgx_plan pgx_plan_t;
@@ -238,7 +239,7 @@ int pgx_compact(pgx_runner_t* h, const uint32_t* d_verdict, uint32_t n, uint32_t
                 uint32_t* d_index, uint32_t* d_count, void* stream);
 
 /* Autotuner: times persistent launch shapes (blocks_per_cu x waves_per_block x
- * chunk), the plain grid and 2/4/8-lane groups on bucket b with `flags`, keeps
+ * chunk), the plain grid and 2/4/8/16/32-lane groups on bucket b with `flags`, keeps
  * the fastest for b's kernel in the
  * runner (used when tuning is NULL) and returns it in *best with the measured
  * candidates per second in *per_sec (either may be NULL). */
@@ -313,7 +314,7 @@ typedef struct {
   uint32_t stage_columns;/* columns admitted per class per iteration (pinned staging) */
   uint32_t group_min_bits;  /* classes at least this wide (kernel capacity, bits) test ... */
   uint32_t group_max_items; /* ... iterations of at most this many candidates with ... */
-  uint32_t group_lanes;     /* ... 2, 4 or 8 lanes per candidate (lower latency for the wide
+  uint32_t group_lanes;     /* ... 2, 4, 8, 16 or 32 lanes per candidate (lower latency for the wide
                                classes on a lightly loaded GPU); group_max_items 0 = never */
   uint32_t wide_cu_percent; /* 1..99: the classes at least group_min_bits wide run on this
                                share of the CUs and the narrower ones on the rest, so a
@@ -346,6 +347,12 @@ typedef struct {
   uint64_t bytes_in_use;
   uint64_t tests;
   uint32_t active, waiting, in_flight, finished_unread;
+  /* device time of the iterations that tested candidates (HIP events on the
+   * iteration's stream), and the device's idle time between one iteration's
+   * end and the next one's start on the same stream: the host's scheduling
+   * turnaround (poll, apply, admit, launch) plus any time with no work */
+  uint64_t iterations, gaps;
+  double iter_ms, gap_ms;
 } pgx_serve_stats_t;
 
 typedef struct pgx_server pgx_server_t;

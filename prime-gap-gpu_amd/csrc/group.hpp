@@ -1,14 +1,17 @@
 // Several lanes per candidate: the PRP test of one number spread over a group
-// of G = 2, 4 or 8 consecutive lanes (inside one row of 16), for launches with few
+// of G = 2, 4, 8, 16 or 32 consecutive lanes, for launches with few
 // candidates, where one lane per candidate leaves most SIMDs idle and the
 // latency of a single exponentiation bounds the launch (Miller-Rabin bases on
 // a few survivors, serving iterations of wide classes).
 //
 // A number is Sp = Q*G radix-2^28 limbs; lane r of a group owns limbs
-// r*Q .. r*Q+Q-1 in registers.  Cross-lane traffic is DPP quad permutations
-// (no LDS): broadcasts inside the group, and moves to the next / previous
-// lane (quad permutations for G <= 4; for G = 8 also row shifts by 4 and 1
-// lanes, whose reads across a group boundary callers mask or select away).  The Montgomery product is coarsely integrated operand scanning over
+// r*Q .. r*Q+Q-1 in registers.  Cross-lane traffic is DPP (no LDS memory):
+// broadcasts inside the group (quad permutations for G <= 8, row_newbcast for
+// a row of 16, and for 32 lanes row_newbcast plus row_bcast:15 or an LDS
+// swizzle), and moves to the next / previous lane (quad permutations for G <=
+// 4, row shifts by 1 lane for 8 and 16, wavefront shifts for 32, whose reads
+// across a group boundary callers mask or select away).  The Montgomery
+// product is coarsely integrated operand scanning over
 // the group: for each limb a_i (broadcast from its owner) every lane adds
 // a_i*b_j and m_i*n_j into its 64-bit accumulator slots, the retired position
 // i leaves lane 0, and the window moves down one limb (the slot entering a
@@ -16,7 +19,10 @@
 // them stays in the lane).  Accumulator slots are not normalized during the
 // product: each position takes at most 2*Sp products below 2^56 and one carry
 // below 2^37, so Sp <= 127 keeps them below 2^64.  The result is then
-// normalized with G-1 rounds of carry passing between the lanes.
+// normalized: every lane settles its own slots, the carries move up one lane,
+// and the single-unit carries left over are resolved for the whole group at
+// once from two ballots (carry generated, lane all ones), as a binary
+// addition of lane masks, instead of G-1 rounds of lane-to-lane passing.
 //
 // Values stay lazy as in the single-lane kernels (mont.hpp): R = 2^(28*Sp) >=
 // 16N, a product of inputs < 4N is < 2N.  The verdict logic is the one of
@@ -45,8 +51,10 @@ __device__ __forceinline__ uint32_t dpp(uint32_t v) {
 }
 
 // DPP row shifts (lanes of a row of 16; lanes shifted in from outside the
-// row read 0).
+// row read 0), row rotations, row broadcasts and wavefront shifts by one lane.
 constexpr int kRowShl1 = 0x101, kRowShl4 = 0x104, kRowShr1 = 0x111, kRowShr4 = 0x114;
+constexpr int kRowRor4 = 0x124, kRowRor8 = 0x128, kWaveShl1 = 0x130, kWaveShr1 = 0x138;
+constexpr int kRowBcast15 = 0x142, kRowNewBcast0 = 0x150;
 
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_row(uint32_t v) {
@@ -60,18 +68,34 @@ __device__ __forceinline__ uint32_t other_quad(uint32_t v) {
   return (threadIdx.x & 4u) ? down : up;
 }
 
+// LDS-crossbar swizzle inside groups of 32 lanes (no LDS memory): source
+// lane ((lane & and_mask) | or_mask) ^ xor_mask.
+template <int AND, int OR, int XOR>
+__device__ __forceinline__ uint32_t swizzle32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, AND | (OR << 5) | (XOR << 10));
+}
+
 // Value of group lane O (compile-time) in every lane of the group.
 template <int G, int O>
 __device__ __forceinline__ uint32_t bcast(uint32_t v) {
-  static_assert(G == 2 || G == 4 || G == 8, "groups of 2, 4 or 8 lanes");
+  static_assert(G == 2 || G == 4 || G == 8 || G == 16 || G == 32,
+                "groups of 2, 4, 8, 16 or 32 lanes");
   if constexpr (G == 4) {
     return dpp<qp(O, O, O, O)>(v);
   } else if constexpr (G == 2) {
     return dpp<qp(O, O, 2 + O, 2 + O)>(v);
-  } else {
+  } else if constexpr (G == 8) {
     const uint32_t q = dpp<qp(O % 4, O % 4, O % 4, O % 4)>(v);  // lane O % 4 of each quad
     const uint32_t o = other_quad(q);
     return ((threadIdx.x & 4u) != 0) == (O >= 4) ? q : o;
+  } else if constexpr (G == 16) {
+    return dpp<kRowNewBcast0 + O>(v);  // the row is the group
+  } else if constexpr (O < 16) {
+    // lane O of each row, then the even row's copy into the odd row above it
+    const uint32_t q = dpp<kRowNewBcast0 + O>(v);
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)q, (int)q, kRowBcast15, 0xA, 0xF, false);
+  } else {
+    return swizzle32<0, O, 0>(v);
   }
 }
 
@@ -79,7 +103,8 @@ __device__ __forceinline__ uint32_t bcast(uint32_t v) {
 // 0 back: callers mask it).
 template <int G>
 __device__ __forceinline__ uint32_t from_next(uint32_t v) {
-  if constexpr (G == 8) return dpp_row<kRowShl1>(v);
+  if constexpr (G == 32) return dpp_row<kWaveShl1>(v);
+  else if constexpr (G == 8 || G == 16) return dpp_row<kRowShl1>(v);
   else if constexpr (G == 4) return dpp<qp(1, 2, 3, 3)>(v);
   else return dpp<qp(1, 1, 3, 3)>(v);
 }
@@ -88,7 +113,8 @@ __device__ __forceinline__ uint32_t from_next(uint32_t v) {
 // back: callers mask it).
 template <int G>
 __device__ __forceinline__ uint32_t from_prev(uint32_t v) {
-  if constexpr (G == 8) return dpp_row<kRowShr1>(v);
+  if constexpr (G == 32) return dpp_row<kWaveShr1>(v);
+  else if constexpr (G == 8 || G == 16) return dpp_row<kRowShr1>(v);
   else if constexpr (G == 4) return dpp<qp(0, 0, 1, 2)>(v);
   else return dpp<qp(0, 0, 2, 2)>(v);
 }
@@ -115,23 +141,42 @@ __device__ __forceinline__ uint32_t group_or(uint32_t v) {
   v |= dpp<qp(1, 0, 3, 2)>(v);
   if constexpr (G >= 4) v |= dpp<qp(2, 3, 0, 1)>(v);
   if constexpr (G == 8) v |= other_quad(v);
+  if constexpr (G >= 16) {  // the four quads of the row
+    v |= dpp<kRowRor4>(v);
+    v |= dpp<kRowRor8>(v);
+  }
+  if constexpr (G == 32) v |= swizzle32<31, 0, 16>(v);  // the other row of the half
   return v;
 }
 
 template <int G>
 __device__ __forceinline__ double group_sum(double v) {
-  v += __longlong_as_double(
-      (long long)dpp64<qp(1, 0, 3, 2)>((uint64_t)__double_as_longlong(v)));
-  if constexpr (G >= 4)
-    v += __longlong_as_double(
-        (long long)dpp64<qp(2, 3, 0, 1)>((uint64_t)__double_as_longlong(v)));
+  auto add = [&](uint64_t o) { v += __longlong_as_double((long long)o); };
+  auto bits = [&]() { return (uint64_t)__double_as_longlong(v); };
+  add(dpp64<qp(1, 0, 3, 2)>(bits()));
+  if constexpr (G >= 4) add(dpp64<qp(2, 3, 0, 1)>(bits()));
   if constexpr (G == 8) {
-    const uint64_t u = (uint64_t)__double_as_longlong(v);
-    const uint64_t o = (uint64_t)other_quad((uint32_t)u) |
-                       ((uint64_t)other_quad((uint32_t)(u >> 32)) << 32);
-    v += __longlong_as_double((long long)o);
+    const uint64_t u = bits();
+    add((uint64_t)other_quad((uint32_t)u) | ((uint64_t)other_quad((uint32_t)(u >> 32)) << 32));
+  }
+  if constexpr (G >= 16) {
+    add(dpp64<kRowRor4>(bits()));
+    add(dpp64<kRowRor8>(bits()));
+  }
+  if constexpr (G == 32) {
+    const uint64_t u = bits();
+    add((uint64_t)swizzle32<31, 0, 16>((uint32_t)u) |
+        ((uint64_t)swizzle32<31, 0, 16>((uint32_t)(u >> 32)) << 32));
   }
   return v;
+}
+
+// Lanes that are the top lane of their group, as a wavefront lane mask.
+template <int G>
+constexpr uint64_t top_lanes() {
+  uint64_t m = 0;
+  for (int i = G - 1; i < 64; i += G) m |= 1ull << i;
+  return m;
 }
 
 template <int G>
@@ -170,7 +215,13 @@ __device__ __forceinline__ int64_t settle(uint32_t (&r)[Q], const int64_t (&t)[Q
 }
 
 // Unsigned accumulator slots -> normalized limbs (the carry out is zero for
-// every value this file produces).
+// every value this file produces).  Every lane settles its slots, the carry
+// of each lane (< 2^38) enters the lane above, which leaves carries of 0 or 1
+// (Q >= 2 limbs of 28 bits absorb 2^38).  Those are resolved at once: a carry
+// reaches lane r when some lane below it generated one and every lane in
+// between is all ones, i.e. the carries into the lanes are ((g << 1) + p) ^ p
+// over the ballot masks g (generated) and p (all ones), neither taken from a
+// group's top lane so that nothing crosses into the next group.
 template <int Q, int G>
 __device__ __forceinline__ void settle_u(uint32_t (&r)[Q], const uint64_t (&t)[Q]) {
   const uint32_t lane = lane_id<G>();
@@ -180,6 +231,31 @@ __device__ __forceinline__ void settle_u(uint32_t (&r)[Q], const uint64_t (&t)[Q
     const uint64_t v = t[s] + c;
     r[s] = (uint32_t)v & MASK;
     c = v >> RB;
+  }
+  if constexpr (G >= 4) {
+    static_assert(Q >= 2, "two limbs per lane absorb a carry below 2^38");
+    uint64_t in = (uint64_t)from_prev64<G>((int64_t)c);
+    if (lane == 0) in = 0;
+    uint32_t ones = 0;  // zero when every limb of the lane is all ones
+#pragma unroll
+    for (int s = 0; s < Q; ++s) {
+      const uint64_t v = (uint64_t)r[s] + in;
+      r[s] = (uint32_t)v & MASK;
+      in = v >> RB;
+      ones |= r[s] ^ MASK;
+    }
+    constexpr uint64_t kTop = top_lanes<G>();
+    const uint64_t g = __ballot(in != 0) & ~kTop;
+    const uint64_t p = __ballot(ones == 0) & ~kTop;
+    const uint64_t cin = ((g << 1) + p) ^ p;
+    uint32_t k = (uint32_t)(cin >> (threadIdx.x & 63u)) & 1u;
+#pragma unroll
+    for (int s = 0; s < Q; ++s) {
+      const uint32_t v = r[s] + k;
+      r[s] = v & MASK;
+      k = v >> RB;
+    }
+    return;
   }
 #pragma unroll
   for (int round = 1; round < G; ++round) {
@@ -339,29 +415,44 @@ __device__ __forceinline__ void mul_small(uint32_t (&x)[Q], uint32_t a, const ui
 }
 
 // Exponentiation of x = Mont(base) by N-1 with the strong-test checks (the
-// group form of powm_verdict in prp.hpp); one = R mod N from the caller.
-// res <- base^(N-1) mod N.
+// group form of powm_verdict in prp.hpp, with the same wavefront-uniform bit
+// index); one = R mod N from the caller.  res <- base^(N-1) mod N.
 template <int Q, int G, class WordFn>
 __device__ __forceinline__ uint32_t powm_verdict(uint32_t (&x)[Q], const uint32_t (&one)[Q],
                                                  const uint32_t (&n)[Q], uint32_t ninv, int L,
                                                  int t, uint32_t base, double inv_n, WordFn word,
                                                  uint32_t (&res)[Q]) {
-  bool sprp = false;
+  uint32_t sprp = 0;
   if (t == L - 1) {
     canon<Q, G>(x, n);
-    sprp = equal<Q, G>(x, one) || is_neg<Q, G>(x, one, n);
+    sprp = (equal<Q, G>(x, one) || is_neg<Q, G>(x, one, n)) ? 1u : 0u;
   }
-  uint32_t w = 0;
-  for (int p = L - 2; p >= 0; --p) {
-    if (p == L - 2 || (p & 31) == 31) w = word(p >> 5);
-    const uint32_t bit = (p > 0) ? ((w >> (p & 31)) & 1u) : 0u;
-    mont_mul<Q, G>(x, x, x, n, ninv);
-    if (base == 2u) dbl_lazy<Q, G>(x, bit);
-    else if (bit) mul_small<Q, G>(x, base, n, inv_n, L);
-    if (p <= t && p >= 1) {
-      canon<Q, G>(x, n);
-      sprp = sprp || is_neg<Q, G>(x, one, n) || (p == t && equal<Q, G>(x, one));
-    }
+  // every lane of a group holds the same L and t
+  const int Lw = (int)wave_max<12>((uint32_t)L);
+  const int tw = (int)wave_max<12>((uint32_t)t);
+  if (L != Lw) {
+#pragma unroll
+    for (int s = 0; s < Q; ++s) x[s] = one[s];
+  }
+  const int top = (Lw - 2) >> 5;
+  uint32_t w_next = word(top);
+  for (int wi = top; wi >= 0; --wi) {
+    const uint32_t w = (wi == 0) ? (w_next & ~1u) : w_next;
+    if (wi > 0) w_next = word(wi - 1);
+    int b = (wi == top) ? ((Lw - 2) & 31) : 31;
+    do {
+      const uint32_t bit = (w >> b) & 1u;
+      mont_mul<Q, G>(x, x, x, n, ninv);
+      if (base == 2u) dbl_lazy<Q, G>(x, bit);
+      else if (bit) mul_small<Q, G>(x, base, n, inv_n, L);
+      const int p = 32 * wi + b;
+      if (p <= tw) {
+        if (p <= t && p >= 1) {
+          canon<Q, G>(x, n);
+          if (is_neg<Q, G>(x, one, n) || (p == t && equal<Q, G>(x, one))) sprp = 1u;
+        }
+      }
+    } while (--b >= 0);
   }
   uint32_t unit[Q];
 #pragma unroll

@@ -136,6 +136,8 @@ struct RegisterGroups {
     PrpLaunchFn* t2 = group_table(2);
     PrpLaunchFn* t4 = group_table(4);
     PrpLaunchFn* t8 = group_table(8);
+    PrpLaunchFn* t16 = group_table(16);
+    PrpLaunchFn* t32 = group_table(32);
     t2[16] = &launch_group<8, 2>;
     t2[24] = &launch_group<12, 2>;
     t2[32] = &launch_group<16, 2>;
@@ -153,6 +155,12 @@ struct RegisterGroups {
     t8[48] = &launch_group<6, 8>;
     t8[64] = &launch_group<8, 8>;
     t8[80] = &launch_group<10, 8>;
+    t16[32] = &launch_group<2, 16>;
+    t16[48] = &launch_group<3, 16>;
+    t16[64] = &launch_group<4, 16>;
+    t16[80] = &launch_group<5, 16>;
+    t32[64] = &launch_group<2, 32>;
+    t32[96] = &launch_group<3, 32>;
   }
 } register_groups;
 

@@ -66,14 +66,17 @@ inline int kernel_for_bits(uint32_t max_bits) {
   return 0;
 }
 
-// Launchers of the G-lanes-per-candidate kernels (G = 2, 4 or 8), indexed by
-// their radix-2^28 limb count Sp (capacity 28*Sp - 4 bits).
+// Launchers of the G-lanes-per-candidate kernels (G = 2, 4, 8, 16 or 32),
+// indexed by their radix-2^28 limb count Sp = Q * G (capacity 28*Sp - 4 bits).
+constexpr int kMaxGroupLimbs = 96;
 PrpLaunchFn* group_table(int G);
+
+inline bool valid_group(int G) { return G == 2 || G == 4 || G == 8 || G == 16 || G == 32; }
 
 // Smallest instantiated Sp of the G-lane kernels that holds max_bits, or 0.
 inline int group_width_for_bits(uint32_t max_bits, int G) {
-  if (G != 2 && G != 4 && G != 8) return 0;
-  for (int Sp = 1; Sp <= kMaxKernelLimbs; ++Sp)
+  if (!valid_group(G)) return 0;
+  for (int Sp = 1; Sp <= kMaxGroupLimbs; ++Sp)
     if (group_table(G)[Sp] && 28 * Sp - 4 >= (int)max_bits) return Sp;
   return 0;
 }

@@ -31,9 +31,8 @@ int check_tuning(pgx_runner* h, const pgx_tuning_t& t) {
       t.blocks_per_cu > 64 || t.chunk < 64 || t.chunk % 64 || t.chunk > 4096)
     return fail(h, PGX_E_ARG, "bad tuning: blocks_per_cu=%u waves_per_block=%u chunk=%u",
                 t.blocks_per_cu, t.waves_per_block, t.chunk);
-  if (t.lanes_per_number != 1 && t.lanes_per_number != 2 && t.lanes_per_number != 4 &&
-      t.lanes_per_number != 8)
-    return fail(h, PGX_E_ARG, "lanes_per_number=%u: 1, 2, 4 or 8", t.lanes_per_number);
+  if (t.lanes_per_number != 1 && !pgx::valid_group((int)t.lanes_per_number))
+    return fail(h, PGX_E_ARG, "lanes_per_number=%u: 1, 2, 4, 8, 16 or 32", t.lanes_per_number);
   return PGX_OK;
 }
 
@@ -237,7 +236,7 @@ int pgx_autotune(pgx_runner_t* h, const pgx_bucket_t* b, uint32_t flags, pgx_tun
           top = t;
         }
       }
-  for (uint32_t lanes : {1u, 2u, 4u, 8u}) {  // the plain grid, then lane groups
+  for (uint32_t lanes : {1u, 2u, 4u, 8u, 16u, 32u}) {  // the plain grid, then lane groups
     if (rc != PGX_OK) break;
     if (lanes > 1 && !pgx::group_width_for_bits(b->max_bits, (int)lanes)) continue;
     pgx_tuning_t t = default_tuning();

@@ -31,10 +31,18 @@ BasesLaunchFn* bases_table() {
 }
 
 PrpLaunchFn* group_table(int G) {
-  static PrpLaunchFn two[kMaxKernelLimbs + 1] = {};
-  static PrpLaunchFn four[kMaxKernelLimbs + 1] = {};
-  static PrpLaunchFn eight[kMaxKernelLimbs + 1] = {};
-  return G == 2 ? two : G == 4 ? four : eight;
+  static PrpLaunchFn two[kMaxGroupLimbs + 1] = {};
+  static PrpLaunchFn four[kMaxGroupLimbs + 1] = {};
+  static PrpLaunchFn eight[kMaxGroupLimbs + 1] = {};
+  static PrpLaunchFn sixteen[kMaxGroupLimbs + 1] = {};
+  static PrpLaunchFn thirtytwo[kMaxGroupLimbs + 1] = {};
+  switch (G) {
+    case 2: return two;
+    case 4: return four;
+    case 8: return eight;
+    case 16: return sixteen;
+    default: return thirtytwo;
+  }
 }
 
 // Per thread: runners are created from several host threads at once (one per

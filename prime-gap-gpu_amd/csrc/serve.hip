@@ -117,6 +117,13 @@ struct Slot {
   uint32_t* h_stage = nullptr;  // pinned admission staging, W rows x cap_stage
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  // timing of the iterations that tested candidates: `start` before the
+  // iteration's first copy, `done` after its last; prev_done is the previous
+  // iteration's end on this stream, so prev_done -> start is the time the
+  // device waited for the host to schedule (what a device-side loop removes)
+  hipEvent_t start = nullptr, prev_done = nullptr;
+  bool timed = false;     // the iteration in flight has a start event
+  bool has_prev = false;  // prev_done ends the iteration before it on this stream
   bool in_flight = false;
   std::vector<Segment> segs;
 };
@@ -145,6 +152,8 @@ struct pgx_server {
   std::map<int, std::unique_ptr<Class>> classes;
   std::vector<pgx_serve_result_t> finished;
   uint64_t tests_total = 0;
+  uint64_t iterations = 0, gaps = 0;  // timed iterations, and gaps measured between them
+  double iter_ms = 0, gap_ms = 0;
 };
 
 namespace {
@@ -152,6 +161,8 @@ namespace {
 void release_class(Class& c) {
   for (Slot& q : c.slots) {
     if (q.done) (void)hipEventDestroy(q.done);
+    if (q.start) (void)hipEventDestroy(q.start);
+    if (q.prev_done) (void)hipEventDestroy(q.prev_done);
     if (q.stream) (void)hipStreamDestroy(q.stream);
     for (uint32_t* p : {q.d_index, q.d_out})
       if (p) (void)hipFree(p);
@@ -211,7 +222,9 @@ int make_class(pgx_server* s, int S, Class** out) {
     for (Slot& q : c->slots) {
       if (e == hipSuccess)
         e = hipExtStreamCreateWithCUMask(&q.stream, (uint32_t)mask.size(), mask.data());
-      if (e == hipSuccess) e = hipEventCreateWithFlags(&q.done, hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventCreate(&q.done);
+      if (e == hipSuccess) e = hipEventCreate(&q.start);
+      if (e == hipSuccess) e = hipEventCreate(&q.prev_done);
     }
   }
   if (e != hipSuccess) {
@@ -275,9 +288,15 @@ int launch_iteration(pgx_server* s, Class& c, Slot& q, bool copies_pending) {
     if (copies_pending) {
       PGX_HIP(s->h, hipEventRecord(q.done, q.stream));
       q.in_flight = true;
+      q.timed = false;
+      q.has_prev = false;  // copies only: no iteration to measure a gap after
     }
     return PGX_OK;
   }
+  if (q.timed) std::swap(q.done, q.prev_done);  // the last iteration's end
+  q.has_prev = q.timed;
+  q.timed = true;
+  PGX_HIP(s->h, hipEventRecord(q.start, q.stream));
   PGX_HIP(s->h, hipMemcpyAsync(q.d_index, q.h_index, (size_t)items * 4, hipMemcpyHostToDevice,
                                q.stream));
   // a small iteration of a wide class: a group of lanes per candidate
@@ -357,6 +376,17 @@ int advance(pgx_server* s, Class& c) {
       if (e != hipSuccess)
         return fail(s->h, PGX_E_HIP, "serving iteration: %s", hipGetErrorString(e));
       q.in_flight = false;
+      if (q.timed) {
+        float busy = 0, gap = 0;
+        if (hipEventElapsedTime(&busy, q.start, q.done) == hipSuccess) {
+          s->iterations += 1;
+          s->iter_ms += busy;
+        }
+        if (q.has_prev && hipEventElapsedTime(&gap, q.prev_done, q.start) == hipSuccess) {
+          s->gaps += 1;
+          s->gap_ms += gap;
+        }
+      }
       apply(s, q);
     }
     if (!free_slot) free_slot = &q;
@@ -378,8 +408,7 @@ int pgx_serve_create(pgx_server_t** out, int device, const pgx_serve_config_t* c
   *out = nullptr;
   if (cfg->chunk == 0 || cfg->max_requests == 0 || cfg->pool_columns == 0 ||
       cfg->max_side == 0 || cfg->max_side * 2ull > cfg->pool_columns ||
-      (cfg->group_max_items && cfg->group_lanes != 2 && cfg->group_lanes != 4 &&
-       cfg->group_lanes != 8) ||
+      (cfg->group_max_items && !pgx::valid_group((int)cfg->group_lanes)) ||
       cfg->wide_cu_percent > 99 || (cfg->wide_cu_percent && !cfg->group_min_bits) ||
       cfg->slots_per_class > 4)
     return fail(nullptr, PGX_E_ARG, "bad serving config");
@@ -471,6 +500,10 @@ int pgx_serve_stats(const pgx_server_t* s, pgx_serve_stats_t* st) {
   st->bytes_in_use = s->in_use;
   st->tests = s->tests_total;
   st->finished_unread = (uint32_t)s->finished.size();
+  st->iterations = s->iterations;
+  st->iter_ms = s->iter_ms;
+  st->gaps = s->gaps;
+  st->gap_ms = s->gap_ms;
   for (auto& kv : s->classes) {
     st->active += (uint32_t)kv.second->active.size();
     st->waiting += (uint32_t)kv.second->waiting.size();

@@ -133,7 +133,9 @@ class ServeResult(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [("bytes_in_use", ctypes.c_uint64), ("tests", ctypes.c_uint64),
                 ("active", ctypes.c_uint32), ("waiting", ctypes.c_uint32),
-                ("in_flight", ctypes.c_uint32), ("finished_unread", ctypes.c_uint32)]
+                ("in_flight", ctypes.c_uint32), ("finished_unread", ctypes.c_uint32),
+                ("iterations", ctypes.c_uint64), ("gaps", ctypes.c_uint64),
+                ("iter_ms", ctypes.c_double), ("gap_ms", ctypes.c_double)]
 
 
 class SearchConfig(ctypes.Structure):
@@ -691,7 +693,7 @@ class Server:
     def stats(self) -> dict:
         st = ServeStats()
         self._check(lib.pgx_serve_stats(self._s, ctypes.byref(st)))
-        return {k: int(getattr(st, k)) for k, _ in st._fields_}
+        return {k: (float if k.endswith("_ms") else int)(getattr(st, k)) for k, _ in st._fields_}
 
     def close(self):
         if getattr(self, "_s", None):

@@ -1,5 +1,5 @@
-"""GPU parity of the kernels that spread one candidate over a group of 2, 4 or 8
-lanes (lanes_per_number in pgx_tuning_t): verdicts of every mode and
+"""GPU parity of the kernels that spread one candidate over a group of 2, 4, 8,
+16 or 32 lanes (lanes_per_number in pgx_tuning_t): verdicts of every mode and
 residues against the GMP oracle and against the one-lane kernels, through
 persistent (bucket) and indexed launches, at every group kernel width."""
 import sys
@@ -36,7 +36,7 @@ def run_bucket(run, L, mode, lanes, residues=True):
     return a.verdicts(), (a.residues() if residues else None)
 
 
-@pytest.mark.parametrize("lanes", [2, 4, 8])
+@pytest.mark.parametrize("lanes", [2, 4, 8, 16, 32])
 def test_group_fermat_residues_all_sizes(lanes):
     with pgx.Runner(1, limbs=16) as run:
         for P, d, words in SIZES:
@@ -48,7 +48,7 @@ def test_group_fermat_residues_all_sizes(lanes):
             assert np.array_equal(r, ores), P
 
 
-@pytest.mark.parametrize("lanes", [2, 4, 8])
+@pytest.mark.parametrize("lanes", [2, 4, 8, 16, 32])
 def test_group_strong_and_reference_bases(lanes):
     with pgx.Runner(1, limbs=16, bases=[2, 3, 5, 7]) as run:
         for P, d, words in SIZES[:4]:
@@ -67,7 +67,7 @@ def test_group_kats_and_tiny_n():
     Ns = list(range(3, 1 << 12, 2)) + [2047, 3277, 1373653, 25326001, 3215031751]
     L = pgx.to_limbs(Ns, 16)
     with pgx.Runner(1, limbs=16, bases=[2, 3, 5, 7]) as run:
-        for lanes in (2, 4, 8):
+        for lanes in (2, 4, 8, 16, 32):
             say(f"lanes={lanes} tiny N and KATs")
             v, r = run_bucket(run, L, pgx.MODE_STRONG, lanes)
             assert np.array_equal(v, O.verdicts(L, O.MODE_STRONG, (2, 3, 5, 7))), lanes
@@ -77,16 +77,16 @@ def test_group_kats_and_tiny_n():
 
 
 def test_group_widest_and_indexed():
-    """2000-bit candidates (the 76-limb four-lane and 80-limb eight-lane
-    kernels) and an indexed launch over a compacted list, equal to the
-    one-lane kernels."""
+    """2000-bit candidates (the 76-limb four-lane, 80-limb eight- and
+    sixteen-lane and 96-limb thirty-two-lane kernels) and an indexed launch
+    over a compacted list, equal to the one-lane kernels."""
     import torch
     rng = np.random.default_rng(7)
     big = [int.from_bytes(rng.bytes(250), "little") | (1 << 1999) | 1 for _ in range(300)]
     L = pgx.to_limbs(big, 64)
     with pgx.Runner(1, limbs=64) as run:
         ov, ores = O.verdicts(L, O.MODE_FERMAT, residues=True)
-        for lanes in (4, 8):
+        for lanes in (4, 8, 16, 32):
             say(f"lanes={lanes} 2000-bit Fermat")
             v, r = run_bucket(run, L, pgx.MODE_FERMAT, lanes)
             assert np.array_equal(v, ov) and np.array_equal(r, ores), lanes
@@ -102,7 +102,7 @@ def test_group_widest_and_indexed():
         fermat.compact(a.verdict.data_ptr(), len(L), pgx.FLAG_FERMAT, index.data_ptr(),
                        count.data_ptr(), stream=st)
         out = {}
-        for lanes in (1, 2, 4, 8):
+        for lanes in (1, 2, 4, 8, 16, 32):
             vv = torch.full((len(L),), -1, dtype=torch.int32, device="cuda")
             torch.cuda.synchronize()
             b = pgx.Bucket(a.cand.data_ptr(), a.ld, a.limbs, len(L), a.max_bits, vv.data_ptr(),
@@ -116,7 +116,7 @@ def test_group_widest_and_indexed():
         surv = np.sort(index.cpu().numpy()[:k].view(np.uint32))
         assert k == int(((a.verdicts() & pgx.FLAG_FERMAT) != 0).sum())
         want = O.verdicts(L[surv], O.MODE_STRONG, (3, 5))
-        for lanes in (1, 2, 4, 8):
+        for lanes in (1, 2, 4, 8, 16, 32):
             assert np.array_equal(out[lanes][surv], want), lanes
             rest = np.setdiff1d(np.arange(len(L)), surv)
             assert np.all(out[lanes][rest] == 0xFFFFFFFF), lanes  # untouched columns
@@ -128,7 +128,7 @@ def test_group_autotune_and_plan():
         a = pgx.DeviceBatch(L)
         t, rate = run.autotune(a.bucket(), pgx.MODE_FERMAT)
         say(f"autotune over lanes: {t.as_dict()} {rate / 1e6:.2f} M/s")
-        assert t.lanes_per_number in (1, 2, 4, 8) and rate > 0
+        assert t.lanes_per_number in (1, 2, 4, 8, 16, 32) and rate > 0
         b = pgx.DeviceBatch(L)
         plan = pgx.Plan(run, [b.bucket()], pgx.MODE_FERMAT, residues=True,
                         tunings=[pgx.Tuning(8, 4, 64, 4)])
@@ -139,5 +139,6 @@ def test_group_autotune_and_plan():
             ov, ores = O.verdicts(L, O.MODE_FERMAT, residues=True)
             assert np.array_equal(b.verdicts(), ov) and np.array_equal(b.residues(), ores)
         plan.close()
-        with pytest.raises(pgx.PgxError):
-            run.test_persistent(a.bucket(), pgx.MODE_FERMAT, tuning=pgx.Tuning(8, 4, 64, 3))
+        for bad in (3, 64):
+            with pytest.raises(pgx.PgxError):
+                run.test_persistent(a.bucket(), pgx.MODE_FERMAT, tuning=pgx.Tuning(8, 4, 64, bad))
