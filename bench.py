@@ -429,6 +429,25 @@ def bench_rounds(cands: np.ndarray, stream, n: int = 1 << 18) -> dict:
                 out[f"pgx_run_rounds{rounds}_batch{batch}_tests_per_s"] = (
                     n / (time.perf_counter() - t0))
                 verdicts[(rounds, batch)] = np.concatenate(got)
+                if batch == BATCH and rounds > 1:
+                    # two batches in flight on the runner: batch k+1 is submitted
+                    # before batch k is waited for, so its first base runs beside
+                    # batch k's later bases
+                    starts = list(range(0, n, batch))
+                    run.submit(cands[:batch], pgx.MODE_REFERENCE)  # the second slot's
+                    run.submit(cands[:batch], pgx.MODE_REFERENCE)  # buffers, made once
+                    run.wait()
+                    run.wait()
+                    t0 = time.perf_counter()
+                    got2 = []
+                    run.submit(cands[:batch], pgx.MODE_REFERENCE)
+                    for k in range(1, len(starts) + 1):
+                        if k < len(starts):
+                            run.submit(cands[starts[k]:starts[k] + batch], pgx.MODE_REFERENCE)
+                        got2.append(run.wait())
+                    out[f"pgx_submit_two_in_flight_rounds{rounds}_batch{batch}_tests_per_s"] = (
+                        n / (time.perf_counter() - t0))
+                    verdicts[(rounds, batch, 2)] = np.concatenate(got2)
     batches = [pgx.DeviceBatch(cands[c0:c0 + BATCH], residues=False) for c0 in range(0, n, BATCH)]
     for rounds in (1, 7):
         with pgx.Runner(1, rounds=rounds, limbs=LIMBS) as run:
@@ -444,7 +463,8 @@ def bench_rounds(cands: np.ndarray, stream, n: int = 1 << 18) -> dict:
             torch.cuda.synchronize()
             one = np.concatenate([b.verdicts() for b in batches])
             out["rounds7_verdicts_equal"] = bool(
-                np.array_equal(one, verdicts[(7, BATCH)]) and np.array_equal(one, verdicts[(7, n)]))
+                np.array_equal(one, verdicts[(7, BATCH)]) and np.array_equal(one, verdicts[(7, n)])
+                and np.array_equal(one, verdicts[(7, BATCH, 2)]))
     out["passed_first_base"] = int((verdicts[(7, n)] >= 1).sum())
     out["passed_all_7"] = int((verdicts[(7, n)] == 7).sum())
     return out
@@ -944,6 +964,8 @@ def summary(plain_512: float, frac_512, extra: dict) -> dict:
         "serving_p50_ms": get("serving", "p50_ms"),
         "serving_p99_ms": get("serving", "p99_ms"),
         "tests_per_s_512_7_bases_pgx_run": get("rounds", "pgx_run_rounds7_batch65536_tests_per_s"),
+        "tests_per_s_512_7_bases_two_in_flight": get(
+            "rounds", "pgx_submit_two_in_flight_rounds7_batch65536_tests_per_s"),
         "tests_per_s_512_from_host_three_threads": get("host_boundary",
                                                        "pgx_run_three_threads_tests_per_s"),
         "tests_per_s_512_dropin_three_gpu_threads": get("dropin", "tests_per_s"),

@@ -57,8 +57,12 @@ int pgx_runner_destroy(pgx_runner_t* h);
 
 /* Copy n candidates (candidate-major limbs, see above) into pinned staging,
  * send them to the device as they are (a device kernel makes them
- * limb-interleaved) and launch asynchronously on the runner's stream.  The
- * caller's array may be reused as soon as this returns.  With several bases
+ * limb-interleaved) and launch asynchronously.  The caller's array may be
+ * reused as soon as this returns.  Up to two batches may be in flight on one
+ * runner (each with its own staging, buffers and stream, so the second runs
+ * beside the first); a third submit is PGX_E_STATE until a pgx_wait, which
+ * returns the batches in submit order.  A runner is used by one host thread
+ * at a time (the reference keeps one runner per GPU thread).  With several bases
  * in the reference or strong mode the first base runs on every candidate and
  * the others, side by side, on the candidates that passed it (compacted on
  * the device); pgx_test_device runs every base in one pass.  The verdicts are
@@ -67,7 +71,7 @@ int pgx_runner_destroy(pgx_runner_t* h);
  * asserts / exits, miller_rabin.h:46-49, 375-376). */
 int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t flags);
 
-/* Wait for the submitted batch.  verdict[i] (n entries): the mode's verdict
+/* Wait for the oldest batch in flight.  verdict[i] (n entries): the mode's verdict
  * word (reference mode: leading bases passed, compare with nbases as
  * miller_rabin.h:406 does).  residues (n * limbs words, candidate-major) is
  * filled when PGX_WANT_RESIDUE was submitted, else may be NULL. */

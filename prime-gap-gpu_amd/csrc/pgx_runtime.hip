@@ -130,31 +130,79 @@ hipError_t launch_transpose(bool to_cols, const uint32_t* src, uint32_t* dst, si
 
 // Queue the D2H copies of a finished batch: verdicts, and residues turned
 // candidate-major on the device (into d_cand, free once the test has run).
-int queue_results(pgx_runner* h, size_t n, uint32_t flags) {
-  PGX_HIP(h, hipMemcpyAsync(h->h_verdict, h->d_verdict, n * 4, hipMemcpyDeviceToHost, h->stream));
+int queue_results(pgx_runner* h, pgx_batch& b, size_t n, uint32_t flags) {
+  PGX_HIP(h, hipMemcpyAsync(b.h_verdict, b.d_verdict, n * 4, hipMemcpyDeviceToHost, b.stream));
   if (flags & PGX_WANT_RESIDUE) {
-    PGX_HIP(h, launch_transpose(false, h->d_res, h->d_cand, n, h->limbs, (uint32_t)n, h->stream));
-    PGX_HIP(h, hipMemcpyAsync(h->h_res, h->d_cand, (size_t)h->limbs * n * 4,
-                              hipMemcpyDeviceToHost, h->stream));
+    PGX_HIP(h, launch_transpose(false, b.d_res, b.d_cand, n, h->limbs, (uint32_t)n, b.stream));
+    PGX_HIP(h, hipMemcpyAsync(b.h_res, b.d_cand, (size_t)h->limbs * n * 4,
+                              hipMemcpyDeviceToHost, b.stream));
   }
   return PGX_OK;
 }
 
+// Staging and device buffers of a batch slot (its stream and count are set by
+// the caller).
+hipError_t alloc_batch(pgx_runner* h, pgx_batch& b) {
+  const size_t words = (size_t)h->limbs * h->max_n;
+  hipError_t e = hipHostMalloc((void**)&b.h_cand, words * 4, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b.h_verdict, h->max_n * 4, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b.h_res, words * 4, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void**)&b.d_cand, words * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&b.d_verdict, h->max_n * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&b.d_res, words * 4);
+  if (e == hipSuccess && h->bases.count > 1)
+    e = hipMalloc((void**)&b.d_survivors, 2 * h->max_n * 4);
+  return e;
+}
+
+// Frees a batch slot's buffers (not its stream or count).
+void release_batch(pgx_batch& b) {
+  if (b.h_cand) (void)hipHostFree(b.h_cand);
+  if (b.h_verdict) (void)hipHostFree(b.h_verdict);
+  if (b.h_res) (void)hipHostFree(b.h_res);
+  if (b.d_cand) (void)hipFree(b.d_cand);
+  if (b.d_verdict) (void)hipFree(b.d_verdict);
+  if (b.d_res) (void)hipFree(b.d_res);
+  if (b.d_survivors) (void)hipFree(b.d_survivors);
+  if (b.d_K) (void)hipFree(b.d_K);
+  if (b.d_m) (void)hipFree(b.d_m);
+  if (b.d_status) (void)hipFree(b.d_status);
+  if (b.h_m) (void)hipHostFree(b.h_m);
+  if (b.h_status) (void)hipHostFree(b.h_status);
+}
+
 void release(pgx_runner* h) {
+  if (h->b1) {
+    if (h->b1->stream) (void)hipStreamDestroy(h->b1->stream);
+    if (h->b1->d_count) (void)hipFree(h->b1->d_count);
+    release_batch(*h->b1);
+  }
+  release_batch(h->b0);  // b0's stream is h->stream, its count a counter slot
   if (h->stream) (void)hipStreamDestroy(h->stream);
-  if (h->h_cand) (void)hipHostFree(h->h_cand);
-  if (h->h_verdict) (void)hipHostFree(h->h_verdict);
-  if (h->h_res) (void)hipHostFree(h->h_res);
-  if (h->d_cand) (void)hipFree(h->d_cand);
-  if (h->d_verdict) (void)hipFree(h->d_verdict);
-  if (h->d_res) (void)hipFree(h->d_res);
   if (h->d_counters) (void)hipFree(h->d_counters);
-  if (h->d_survivors) (void)hipFree(h->d_survivors);
-  if (h->d_K) (void)hipFree(h->d_K);
-  if (h->d_m) (void)hipFree(h->d_m);
-  if (h->d_status) (void)hipFree(h->d_status);
-  if (h->h_m) (void)hipHostFree(h->h_m);
-  if (h->h_status) (void)hipHostFree(h->h_status);
+}
+
+// A free batch slot, in submit order: slot 0, or the second slot (made here
+// the first time slot 0 is busy); NULL when both hold a batch.
+pgx_batch* acquire_batch(pgx_runner* h) {
+  if (!h->b0.pending) return &h->b0;
+  if (!h->b1) {
+    auto b = std::make_unique<pgx_batch>();
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_count, 4);
+    if (e == hipSuccess) e = alloc_batch(h, *b);
+    if (e != hipSuccess) {
+      if (b->stream) (void)hipStreamDestroy(b->stream);
+      if (b->d_count) (void)hipFree(b->d_count);
+      release_batch(*b);
+      fail(h, PGX_E_HIP, "second batch slot: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    h->b1 = std::move(b);
+  }
+  if (!h->b1->pending) return h->b1.get();
+  fail(h, PGX_E_STATE, "two batches are already in flight");
+  return nullptr;
 }
 
 }  // namespace
@@ -192,20 +240,20 @@ int launch_plain_bases(pgx_runner* h, const uint32_t* d_cand, uint32_t ld, uint3
   return PGX_OK;
 }
 
-// The runner's own batch in d_cand (pgx_submit, pgx_submit_mk): one pass when
-// the runner has one base (or in FERMAT mode), else the first base and then
+// A batch in b.d_cand (pgx_submit, pgx_submit_mk): one pass when the runner
+// has one base (or in FERMAT mode), else the first base and then
 // launch_later_bases.
-int launch_batch(pgx_runner* h, size_t n, uint32_t max_bits, uint32_t flags) {
+int launch_batch(pgx_runner* h, pgx_batch& b, size_t n, uint32_t max_bits, uint32_t flags) {
   const uint32_t mode = flags & PGX_MODE_MASK;
   if (h->bases.count == 1 || mode == PGX_MODE_FERMAT)
-    return launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, true,
-                              h->bases, nullptr, nullptr, h->d_verdict, h->d_res, (uint32_t)n,
-                              h->stream);
-  int rc = launch_plain_bases(h, h->d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, true,
-                              pgx::first_base(h->bases), nullptr, nullptr, h->d_verdict, h->d_res,
-                              (uint32_t)n, h->stream);
+    return launch_plain_bases(h, b.d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, true,
+                              h->bases, nullptr, nullptr, b.d_verdict, b.d_res, (uint32_t)n,
+                              b.stream);
+  int rc = launch_plain_bases(h, b.d_cand, (uint32_t)n, h->limbs, n, max_bits, flags, true,
+                              pgx::first_base(h->bases), nullptr, nullptr, b.d_verdict, b.d_res,
+                              (uint32_t)n, b.stream);
   if (rc != PGX_OK) return rc;
-  return pgx::launch_later_bases(h, (uint32_t)n, max_bits, mode, h->stream);
+  return pgx::launch_later_bases(h, b, (uint32_t)n, max_bits, mode, b.stream);
 }
 
 }  // namespace
@@ -249,20 +297,19 @@ hipError_t pgx::launch_survivor_bases(const uint32_t* cand, uint32_t ld, uint32_
   return pgx::launch_settle(verdict, index, count, n, mask, stream);
 }
 
-int pgx::launch_later_bases(pgx_runner* h, uint32_t n, uint32_t max_bits, uint32_t mode,
-                            hipStream_t stream) {
+int pgx::launch_later_bases(pgx_runner* h, pgx_batch& b, uint32_t n, uint32_t max_bits,
+                            uint32_t mode, hipStream_t stream) {
   if (h->bases.count < 2 || n == 0) return PGX_OK;
   const int S = pgx::kernel_for_bits(max_bits);
-  if (S == 0 || !h->d_survivors)
+  if (S == 0 || !b.d_survivors)
     return fail(h, PGX_E_ARG, "no survivor pass for %u-bit candidates", max_bits);
   pgx::Bases later{};
   later.count = h->bases.count - 1;
   for (uint32_t k = 1; k < h->bases.count; ++k) later.a[k - 1] = h->bases.a[k];
-  // count in counter slot 0 (persistent launches take slots 1..), the
-  // survivor list and the masks in d_survivors
-  PGX_HIP(h, pgx::launch_survivor_bases(h->d_cand, n, h->limbs, 0u, n, S, mode, later,
-                                        h->d_verdict, h->d_survivors, h->d_counters,
-                                        h->d_survivors + h->max_n, stream));
+  // the survivor list and the masks in d_survivors, the count in d_count
+  PGX_HIP(h, pgx::launch_survivor_bases(b.d_cand, n, h->limbs, 0u, n, S, mode, later,
+                                        b.d_verdict, b.d_survivors, b.d_count,
+                                        b.d_survivors + h->max_n, stream));
   return PGX_OK;
 }
 
@@ -309,19 +356,14 @@ int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t lim
   } else {
     pgx::first_primes(nbases, h->bases.a);
   }
-  const size_t words = (size_t)limbs * max_n;
   hipError_t e = hipSetDevice(device);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_cand, words * 4, hipHostMallocDefault);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_verdict, max_n * 4, hipHostMallocDefault);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_res, words * 4, hipHostMallocDefault);
-  if (e == hipSuccess) e = hipMalloc((void**)&h->d_cand, words * 4);
-  if (e == hipSuccess) e = hipMalloc((void**)&h->d_verdict, max_n * 4);
-  if (e == hipSuccess) e = hipMalloc((void**)&h->d_res, words * 4);
+  if (e == hipSuccess) e = alloc_batch(h, h->b0);
   if (e == hipSuccess) e = hipMalloc((void**)&h->d_counters, pgx::kCounters * 4);
-  if (e == hipSuccess && nbases > 1) e = hipMalloc((void**)&h->d_survivors, 2 * max_n * 4);
   if (e == hipSuccess)
     e = hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  h->b0.stream = h->stream;
+  h->b0.d_count = h->d_counters;  // slot 0: persistent launches take slots 1..
   if (e != hipSuccess) {
     release(h);
     delete h;
@@ -335,19 +377,28 @@ int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t lim
 int pgx_runner_destroy(pgx_runner_t* h) {
   if (!h) return PGX_E_ARG;
   (void)hipSetDevice(h->device);
-  if (h->pending) (void)hipStreamSynchronize(h->stream);
+  for (pgx_batch* b : h->inflight) (void)hipStreamSynchronize(b->stream);
   release(h);
   delete h;
   return PGX_OK;
 }
 
+// Marks b as the newest batch in flight.
+void mark_pending(pgx_runner* h, pgx_batch& b, size_t n, uint32_t flags) {
+  b.pending = true;
+  b.pending_n = n;
+  b.pending_flags = flags;
+  h->inflight.push_back(&b);
+}
+
 int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t flags) {
   if (!h) return PGX_E_ARG;
-  if (h->pending) return fail(h, PGX_E_STATE, "a batch is already in flight");
+  PGX_HIP(h, hipSetDevice(h->device));
+  pgx_batch* bp = acquire_batch(h);
+  if (!bp) return h->err.find("second batch") == 0 ? PGX_E_HIP : PGX_E_STATE;
+  pgx_batch& b = *bp;
   if (n == 0) {  // the reference returns early on an empty batch (miller_rabin.h:365-366)
-    h->pending = true;
-    h->pending_n = 0;
-    h->pending_flags = flags;
+    mark_pending(h, b, 0, flags);
     return PGX_OK;
   }
   if (!limbs_le) return fail(h, PGX_E_ARG, "limbs_le is NULL");
@@ -368,33 +419,31 @@ int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t fla
   // Candidate-major through pinned staging; d_res holds the rows until the
   // transpose has read them (the test writes its residues there afterwards,
   // in stream order).
-  std::memcpy(h->h_cand, limbs_le, (size_t)S * n * 4);
-  PGX_HIP(h, hipSetDevice(h->device));
-  PGX_HIP(h, hipMemcpyAsync(h->d_res, h->h_cand, (size_t)S * n * 4, hipMemcpyHostToDevice,
-                            h->stream));
-  PGX_HIP(h, launch_transpose(true, h->d_res, h->d_cand, n, S, (uint32_t)n, h->stream));
-  int rc = launch_batch(h, n, max_bits, flags);
+  std::memcpy(b.h_cand, limbs_le, (size_t)S * n * 4);
+  PGX_HIP(h, hipMemcpyAsync(b.d_res, b.h_cand, (size_t)S * n * 4, hipMemcpyHostToDevice,
+                            b.stream));
+  PGX_HIP(h, launch_transpose(true, b.d_res, b.d_cand, n, S, (uint32_t)n, b.stream));
+  int rc = launch_batch(h, b, n, max_bits, flags);
   if (rc != PGX_OK) return rc;
-  rc = queue_results(h, n, flags);
+  rc = queue_results(h, b, n, flags);
   if (rc != PGX_OK) return rc;
-  h->pending = true;
-  h->pending_n = n;
-  h->pending_flags = flags;
+  mark_pending(h, b, n, flags);
   return PGX_OK;
 }
 
 int pgx_submit_mk(pgx_runner_t* h, const uint32_t* K, uint32_t k_words, const uint64_t* m,
                   size_t n, uint32_t x, int32_t sign, uint32_t flags) {
   if (!h) return PGX_E_ARG;
-  if (h->pending) return fail(h, PGX_E_STATE, "a batch is already in flight");
   if (!K || k_words == 0 || (n && !m) || (sign != 1 && sign != -1))
     return fail(h, PGX_E_ARG, "bad pgx_submit_mk arguments");
   if (n > h->max_n) return fail(h, PGX_E_CAPACITY, "tests(%zu) > n(%zu)", n, h->max_n);
   if ((flags & PGX_MODE_MASK) > PGX_MODE_FERMAT) return fail(h, PGX_E_ARG, "bad mode");
+  PGX_HIP(h, hipSetDevice(h->device));
+  pgx_batch* bp = acquire_batch(h);
+  if (!bp) return h->err.find("second batch") == 0 ? PGX_E_HIP : PGX_E_STATE;
+  pgx_batch& b = *bp;
   if (n == 0) {
-    h->pending = true;
-    h->pending_n = 0;
-    h->pending_flags = flags;
+    mark_pending(h, b, 0, flags);
     return PGX_OK;
   }
   // Width bound: bits(m*K + x) <= bits(K) + bits(max m) + 1.
@@ -404,54 +453,54 @@ int pgx_submit_mk(pgx_runner_t* h, const uint32_t* K, uint32_t k_words, const ui
   const uint32_t mb = mmax ? 64u - __builtin_clzll(mmax) : 0u;
   const uint32_t bound = std::max<uint32_t>(kb + mb + 1, 2u);
   const uint32_t max_bits = std::min<uint32_t>(bound, 32u * h->limbs);
-  PGX_HIP(h, hipSetDevice(h->device));
-  if (!h->d_m) {
-    PGX_HIP(h, hipMalloc((void**)&h->d_m, h->max_n * 8));
-    PGX_HIP(h, hipHostMalloc((void**)&h->h_m, h->max_n * 8, hipHostMallocDefault));
-    PGX_HIP(h, hipMalloc((void**)&h->d_status, 4));
-    PGX_HIP(h, hipHostMalloc((void**)&h->h_status, 4, hipHostMallocDefault));
+  if (!b.d_m) {
+    PGX_HIP(h, hipMalloc((void**)&b.d_m, h->max_n * 8));
+    PGX_HIP(h, hipHostMalloc((void**)&b.h_m, h->max_n * 8, hipHostMallocDefault));
+    PGX_HIP(h, hipMalloc((void**)&b.d_status, 4));
+    PGX_HIP(h, hipHostMalloc((void**)&b.h_status, 4, hipHostMallocDefault));
   }
-  if (h->h_K.size() != k_words || std::memcmp(h->h_K.data(), K, (size_t)k_words * 4) != 0) {
-    // K changed: upload it once (the stream is idle, no batch is in flight)
-    if (h->d_K) (void)hipFree(h->d_K);
-    h->d_K = nullptr;
-    h->h_K.clear();
-    PGX_HIP(h, hipMalloc((void**)&h->d_K, (size_t)k_words * 4));
-    PGX_HIP(h, hipMemcpy(h->d_K, K, (size_t)k_words * 4, hipMemcpyHostToDevice));
-    h->h_K.assign(K, K + k_words);
+  if (b.h_K.size() != k_words || std::memcmp(b.h_K.data(), K, (size_t)k_words * 4) != 0) {
+    // K changed: upload it once (this slot's stream is idle: its last batch
+    // was waited for)
+    if (b.d_K) (void)hipFree(b.d_K);
+    b.d_K = nullptr;
+    b.h_K.clear();
+    PGX_HIP(h, hipMalloc((void**)&b.d_K, (size_t)k_words * 4));
+    PGX_HIP(h, hipMemcpy(b.d_K, K, (size_t)k_words * 4, hipMemcpyHostToDevice));
+    b.h_K.assign(K, K + k_words);
   }
-  std::memcpy(h->h_m, m, n * 8);
-  PGX_HIP(h, hipMemcpyAsync(h->d_m, h->h_m, n * 8, hipMemcpyHostToDevice, h->stream));
-  PGX_HIP(h, pgx::launch_fill(h->d_status, 1, 0u, h->stream));
-  int rc = pgx_build_candidates(h, h->d_K, k_words, h->d_m, nullptr, x, sign, (uint32_t)n,
-                                h->d_cand, (uint32_t)n, h->limbs, h->d_status, h->stream);
+  std::memcpy(b.h_m, m, n * 8);
+  PGX_HIP(h, hipMemcpyAsync(b.d_m, b.h_m, n * 8, hipMemcpyHostToDevice, b.stream));
+  PGX_HIP(h, pgx::launch_fill(b.d_status, 1, 0u, b.stream));
+  int rc = pgx_build_candidates(h, b.d_K, k_words, b.d_m, nullptr, x, sign, (uint32_t)n,
+                                b.d_cand, (uint32_t)n, h->limbs, b.d_status, b.stream);
   if (rc != PGX_OK) return rc;
-  rc = launch_batch(h, n, max_bits, flags);
+  rc = launch_batch(h, b, n, max_bits, flags);
   if (rc != PGX_OK) return rc;
-  PGX_HIP(h, hipMemcpyAsync(h->h_status, h->d_status, 4, hipMemcpyDeviceToHost, h->stream));
-  rc = queue_results(h, n, flags);
+  PGX_HIP(h, hipMemcpyAsync(b.h_status, b.d_status, 4, hipMemcpyDeviceToHost, b.stream));
+  rc = queue_results(h, b, n, flags);
   if (rc != PGX_OK) return rc;
-  h->pending = true;
-  h->pending_n = n;
-  h->pending_flags = flags;
-  h->status_pending = true;
+  mark_pending(h, b, n, flags);
+  b.status_pending = true;
   return PGX_OK;
 }
 
 int pgx_wait(pgx_runner_t* h, uint32_t* verdict, uint32_t* residues) {
   if (!h) return PGX_E_ARG;
-  if (!h->pending) return fail(h, PGX_E_STATE, "no batch in flight");
-  h->pending = false;
-  const size_t n = h->pending_n;
+  if (h->inflight.empty()) return fail(h, PGX_E_STATE, "no batch in flight");
+  pgx_batch& b = *h->inflight.front();  // the oldest batch
+  h->inflight.pop_front();
+  b.pending = false;
+  const size_t n = b.pending_n;
   if (n == 0) return PGX_OK;
-  PGX_HIP(h, hipStreamSynchronize(h->stream));
-  if (h->status_pending) {
-    h->status_pending = false;
-    if (*h->h_status) return fail(h, PGX_E_INPUT, "a candidate m*K +- x did not fit or was negative");
+  PGX_HIP(h, hipStreamSynchronize(b.stream));
+  if (b.status_pending) {
+    b.status_pending = false;
+    if (*b.h_status) return fail(h, PGX_E_INPUT, "a candidate m*K +- x did not fit or was negative");
   }
-  if (verdict) std::memcpy(verdict, h->h_verdict, n * 4);
-  if ((h->pending_flags & PGX_WANT_RESIDUE) && residues)
-    std::memcpy(residues, h->h_res, (size_t)h->limbs * n * 4);
+  if (verdict) std::memcpy(verdict, b.h_verdict, n * 4);
+  if ((b.pending_flags & PGX_WANT_RESIDUE) && residues)
+    std::memcpy(residues, b.h_res, (size_t)h->limbs * n * 4);
   return PGX_OK;
 }
 

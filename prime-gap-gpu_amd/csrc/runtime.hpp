@@ -5,7 +5,9 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <deque>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -17,12 +19,11 @@ namespace pgx {
 constexpr uint32_t kCounters = 64;  // counter slots per runner
 }  // namespace pgx
 
-struct pgx_runner {
-  int device = 0;
-  int n_cu = 0;  // compute units of the device
-  size_t max_n = 0;
-  uint32_t limbs = 0;
-  pgx::Bases bases{};
+// One batch of pgx_submit / pgx_submit_mk: its staging, device buffers,
+// stream and state.  A runner holds up to two, so that the later bases of one
+// batch overlap the first base of the next (the reference keeps three batches
+// in flight with one runner each, gap_search_gpu.cpp:887-904).
+struct pgx_batch {
   hipStream_t stream = nullptr;
   // pinned staging, candidate-major as at the C ABI (the device transposes)
   uint32_t* h_cand = nullptr;
@@ -32,11 +33,10 @@ struct pgx_runner {
   uint32_t* d_cand = nullptr;
   uint32_t* d_verdict = nullptr;
   uint32_t* d_res = nullptr;
-  uint32_t* d_counters = nullptr;  // work counters of persistent launches
-  pgx::CounterSlots slots{pgx::kCounters};  // the slot each persistent launch takes
   // survivors of the first base (runners with several bases): column list of
-  // max_n words (its length in d_counters[0]), then their per-base pass masks
+  // max_n words (its length in *d_count), then their per-base pass masks
   uint32_t* d_survivors = nullptr;
+  uint32_t* d_count = nullptr;
   // device construction (pgx_submit_mk): K cache, multipliers, status word
   std::vector<uint32_t> h_K;  // the K held in d_K
   uint32_t* d_K = nullptr;
@@ -45,10 +45,23 @@ struct pgx_runner {
   uint32_t* d_status = nullptr;
   uint32_t* h_status = nullptr;  // pinned
   bool status_pending = false;   // the pending batch was built on the device
-  // pending batch
   bool pending = false;
   size_t pending_n = 0;
   uint32_t pending_flags = 0;
+};
+
+struct pgx_runner {
+  int device = 0;
+  int n_cu = 0;  // compute units of the device
+  size_t max_n = 0;
+  uint32_t limbs = 0;
+  pgx::Bases bases{};
+  hipStream_t stream = nullptr;    // the runner's stream (also batch slot 0's)
+  uint32_t* d_counters = nullptr;  // work counters of persistent launches
+  pgx::CounterSlots slots{pgx::kCounters};  // the slot each persistent launch takes
+  pgx_batch b0;                    // slot 0: the m-interval search borrows its buffers
+  std::unique_ptr<pgx_batch> b1;   // the second batch slot, made when first needed
+  std::deque<pgx_batch*> inflight; // submitted batches not yet waited for, oldest first
   // autotuned persistent shapes, keyed by (kernel limb count, mode)
   std::map<uint32_t, pgx_tuning_t> tuned;
   std::string err;
@@ -101,12 +114,11 @@ hipError_t launch_survivor_bases(const uint32_t* cand, uint32_t ld, uint32_t lim
                                  const Bases& later, uint32_t* verdict, uint32_t* index,
                                  uint32_t* count, uint32_t* mask, hipStream_t stream);
 
-// After a first-base launch over columns 0..n-1 of the runner's d_cand (ld = n)
+// After a first-base launch over columns 0..n-1 of batch b's d_cand (ld = n)
 // into d_verdict: the runner's other bases on the columns that passed, one
 // (column, base) pair per lane, and the verdicts of those columns set to the
-// count of leading bases passed.  Uses the runner's survivor buffers and
-// counter slot 0 (one batch at a time per runner).  PGX_E_*.
-int launch_later_bases(pgx_runner* h, uint32_t n, uint32_t max_bits, uint32_t mode,
+// count of leading bases passed.  Uses b's survivor buffers and count.  PGX_E_*.
+int launch_later_bases(pgx_runner* h, pgx_batch& b, uint32_t n, uint32_t max_bits, uint32_t mode,
                        hipStream_t stream);
 
 // Launch over device buffers with the runner's tuned (or default) shape for

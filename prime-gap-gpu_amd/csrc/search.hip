@@ -494,7 +494,7 @@ int pgx_search_create(pgx_search_t** out, pgx_runner_t* h, const uint32_t* K, ui
                       const pgx_search_config_t* cfg) {
   if (!out || !h || !K || !k_words || !cfg) return PGX_E_ARG;
   *out = nullptr;
-  if (h->pending) return pgx::fail(h, PGX_E_STATE, "the runner has a batch in flight");
+  if (!h->inflight.empty()) return pgx::fail(h, PGX_E_STATE, "the runner has a batch in flight");
   const pgx_search_config_t c = *cfg;
   // the reference's own checks (gap_search_gpu.cpp:502-504, gap_common.cpp:431-446)
   if ((c.d & 1u) || c.d == 0 || (c.m_start & 1u) || (c.m_inc & 1u) || c.m_inc < 2 ||
@@ -669,7 +669,7 @@ int pgx_search_destroy(pgx_search_t* s) {
 int pgx_search_sieve(pgx_search_t* s, uint32_t X, uint32_t* n_tests) {
   if (!s) return PGX_E_ARG;
   if (X == 0 || (X & 1u)) return sfail(s, PGX_E_ARG, "X must be even and positive");
-  if (s->runner->pending) return sfail(s, PGX_E_STATE, "the runner has a batch in flight");
+  if (!s->runner->inflight.empty()) return sfail(s, PGX_E_STATE, "the runner has a batch in flight");
   PGX_SHIP(s, hipSetDevice(s->runner->device));
   hipStream_t st = s->runner->stream;
   if (s->n_small) {
@@ -702,7 +702,7 @@ int pgx_search_test(pgx_search_t* s, uint32_t* n_found) {
   if (!s) return PGX_E_ARG;
   if (!s->sieved) return sfail(s, PGX_E_STATE, "no sieved X to test");
   pgx_runner* h = s->runner;
-  if (h->pending) return sfail(s, PGX_E_STATE, "the runner has a batch in flight");
+  if (!h->inflight.empty()) return sfail(s, PGX_E_STATE, "the runner has a batch in flight");
   PGX_SHIP(s, hipSetDevice(h->device));
   hipStream_t st = h->stream;
   const uint32_t found_before = s->h_words[1];
@@ -710,23 +710,23 @@ int pgx_search_test(pgx_search_t* s, uint32_t* n_found) {
   for (uint32_t off = 0; off < s->n_tests; off += batch) {
     const uint32_t nb = std::min(batch, s->n_tests - off);
     PGX_SHIP(s, pgx::launch_build(s->d_K, s->k_words, nullptr, s->d_tests + off,
-                                  s->cfg.m_start, nullptr, s->cur_x, 1, nb, h->d_cand, nb,
+                                  s->cfg.m_start, nullptr, s->cur_x, 1, nb, h->b0.d_cand, nb,
                                   h->limbs, s->d_words + 2, st));
     pgx::PrpArgs pa{};
-    pa.cand = h->d_cand;
+    pa.cand = h->b0.d_cand;
     pa.ld = nb;
     pa.n = nb;
     pa.buf_limbs = h->limbs;
     pa.mode = PGX_MODE_REFERENCE;
     // several bases: the first here, the others side by side on its survivors
     pa.bases = pgx::first_base(h->bases);
-    pa.verdict = h->d_verdict;
+    pa.verdict = h->b0.d_verdict;
     int rc = pgx::launch_auto(h, pa, s->max_bits, st);
-    if (rc == PGX_OK) rc = pgx::launch_later_bases(h, nb, s->max_bits, PGX_MODE_REFERENCE, st);
+    if (rc == PGX_OK) rc = pgx::launch_later_bases(h, h->b0, nb, s->max_bits, PGX_MODE_REFERENCE, st);
     if (rc != PGX_OK) return sfail(s, rc, h->err.c_str());
     hipLaunchKernelGGL(k_consume, dim3((nb + kConsumeBlock - 1) / kConsumeBlock),
                        dim3(kConsumeBlock), 0, st,
-                       h->d_verdict, s->d_tests + off, nb, (uint32_t)h->bases.count, s->cur_x,
+                       h->b0.d_verdict, s->d_tests + off, nb, (uint32_t)h->bases.count, s->cur_x,
                        s->d_F, s->d_found_mi, s->d_found_x, s->d_words + 1);
     PGX_SHIP(s, hipGetLastError());
   }

@@ -23,6 +23,7 @@ from torch.
 """
 from __future__ import annotations
 
+import collections
 import ctypes
 import os
 
@@ -378,6 +379,7 @@ class Runner:
         if rc != PGX_OK:
             raise PgxError(rc, lib.pgx_last_error(None).decode())
         self._h = h
+        self._pending = collections.deque()
 
     def close(self):
         if getattr(self, "_h", None):
@@ -414,14 +416,18 @@ class Runner:
         a = np.ascontiguousarray(limbs_arr, dtype=np.uint32)
         flags = mode | (WANT_RESIDUE if residues else 0)
         self._check(lib.pgx_submit(self._h, _u32(a), a.shape[0], flags))
-        self._pending = (a.shape[0], residues, a)  # only once the library accepted it
+        self._pending.append((a.shape[0], residues, a))  # only once the library accepted it
 
     def wait(self):
-        n, residues, _ = self._pending
+        """Results of the oldest submitted batch (up to two may be in flight)."""
+        if not self._pending:
+            raise PgxError(PGX_E_STATE, "no batch in flight")
+        n, residues, _ = self._pending[0]
         v = np.zeros(n, dtype=np.uint32)
         r = np.zeros((n, self.limbs), dtype=np.uint32) if residues else None
-        self._check(lib.pgx_wait(self._h, _u32(v), _u32(r) if residues else None))
-        self._pending = None
+        rc = lib.pgx_wait(self._h, _u32(v), _u32(r) if residues else None)
+        self._pending.popleft()  # the library retired the batch, failed or not
+        self._check(rc)
         return (v, r) if residues else v
 
     def submit_mk(self, K: int, m: np.ndarray, x: int, sign: int = 1,
@@ -435,7 +441,7 @@ class Runner:
         flags = mode | (WANT_RESIDUE if residues else 0)
         self._check(lib.pgx_submit_mk(self._h, _u32(karr), kw, _u64(marr), len(marr), int(x),
                                       int(sign), flags))
-        self._pending = (len(marr), residues, marr)
+        self._pending.append((len(marr), residues, marr))
 
     def run_mk(self, K: int, m: np.ndarray, x: int, sign: int = 1, mode: int = MODE_REFERENCE,
                residues: bool = False):

@@ -143,12 +143,18 @@ def test_run_test_reference_semantics():
 def test_submit_wait_and_errors():
     with pgx.Runner(8, limbs=2) as run:
         L = pgx.to_limbs([7, 9, 11], 2)
+        L2 = pgx.to_limbs([13, 15], 2)
         run.submit(L, pgx.MODE_FERMAT, residues=True)
+        run.submit(L2, pgx.MODE_FERMAT)  # a second batch in flight
         with pytest.raises(pgx.PgxError) as e:
             run.submit(L, pgx.MODE_FERMAT)
         assert e.value.code == pgx.PGX_E_STATE
-        v, r = run.wait()
+        v, r = run.wait()  # oldest first
         assert list(v & 1) == [1, 0, 1] and pgx.from_limbs(r) == [1, 4, 1]
+        assert list(run.wait() & 1) == [1, 0]
+        with pytest.raises(pgx.PgxError) as e:
+            run.wait()
+        assert e.value.code == pgx.PGX_E_STATE
         for bad in ([8], [1], [0]):
             with pytest.raises(pgx.PgxError) as e:
                 run.run(pgx.to_limbs(bad, 2), pgx.MODE_FERMAT)
@@ -256,3 +262,28 @@ def test_bit_lengths_below_kernel_top(S):
         v, r = run.run(L, pgx.MODE_FERMAT, residues=True)
         assert np.array_equal(v, ov) and np.array_equal(r, orr)
         assert np.array_equal(run.run(L, pgx.MODE_STRONG), O.verdicts(L, O.MODE_STRONG, (2, 3)))
+
+
+def test_two_batches_in_flight_several_bases():
+    """Two batches in flight on one runner (slot 0 and the second slot, each
+    with its own stream), seven bases: the first base of one runs beside the
+    later bases of the other; verdicts equal to one batch at a time, in
+    submit order, through pgx_submit and pgx_submit_mk alike."""
+    m = pgx.gen_multipliers(0x5EED0C01, 10**7, 10**7 + 10**4, 2310, 8)
+    cands, _, _ = pgx.gen_sieved(337, 2310, m, 450000, 2048, 16)
+    a, b = cands[:9000], cands[9000:16001]
+    K = pgx.primorial_over(337, 2310)
+    mm = np.asarray(m, dtype=np.uint64).repeat(600)
+    with pgx.Runner(1 << 14, rounds=7, limbs=16) as run:
+        want_a = run.run(a, pgx.MODE_REFERENCE)
+        want_b = run.run(b, pgx.MODE_REFERENCE)
+        want_mk = run.run_mk(K, mm, 4, 1, pgx.MODE_REFERENCE)
+        for _ in range(3):
+            run.submit(a, pgx.MODE_REFERENCE)
+            run.submit(b, pgx.MODE_REFERENCE)
+            assert np.array_equal(run.wait(), want_a)
+            run.submit_mk(K, mm, 4, 1, pgx.MODE_REFERENCE)
+            assert np.array_equal(run.wait(), want_b)
+            assert np.array_equal(run.wait(), want_mk)
+    assert np.array_equal(want_a, O.verdicts(a, O.MODE_REFERENCE, bases=pgx.first_primes(7)))
+    assert set(np.unique(want_a)) >= {0, 7}
