@@ -59,13 +59,14 @@ def work_per_test(bits: int) -> int:
     return (bits - 1) * (2 * s * s + s)
 
 
+# Kernel limb counts the library instantiates (one-lane kernels).
+KERNEL_LIMBS = list(range(1, 49)) + [56, 64, 74]
+
+
 def kernel_limbs(max_bits: int) -> int:
-    """Radix limbs of the kernel the library picks: the smallest S with
-    RB*S - 4 >= max_bits (RB = 29 up to 31 limbs, 28 above)."""
-    S = 1
-    while (29 if S <= 31 else 28) * S - 4 < max_bits:
-        S += 1
-    return S
+    """Radix limbs of the kernel the library picks: the smallest instantiated S
+    with RB*S - 4 >= max_bits (RB = 29 up to 31 limbs, 28 above)."""
+    return next(S for S in KERNEL_LIMBS if (29 if S <= 31 else 28) * S - 4 >= max_bits)
 
 
 def gen(P: int, d: int, words: int, n_m: int, per_m: int, seed: int) -> np.ndarray:
@@ -126,9 +127,9 @@ MIXED = [(211, 16), (337, 16), (503, 24), (701, 32)]  # (P, buffer words), d = 2
 # script with --copy-probe (profiles/k_prp_plain_s17_p337_d2310_2e20.md):
 # KiB per dispatch of FETCH_SIZE and WRITE_SIZE, each scaled by known bytes /
 # counter reading of the 256 MiB device copy in the same run (copy read
-# 262144 KiB, FETCH_SIZE read 131087.3; copy wrote 262144 KiB, WRITE_SIZE 262144).
+# 262144 KiB, FETCH_SIZE read 131086.94; copy wrote 262144 KiB, WRITE_SIZE 262144).
 TRAFFIC_KIB = {
-    "k_prp_plain<17,true>": {"fetch": 2185.90, "fetch_scale": 262144 / 131087.31,
+    "k_prp_plain<17,true>": {"fetch": 2222.0, "fetch_scale": 262144 / 131086.94,
                              "write": 4352.0, "write_scale": 1.0},
 }
 
@@ -664,6 +665,36 @@ def bench_large(stream, peak: float, reps: int) -> dict:
             ev.record(st)
             stream.wait_event(ev)
 
+    # chunked: Fermat over 65536-candidate chunks on the main stream, each
+    # chunk's survivors compacted and tested with bases 3 and 5 on the side
+    # streams while the next chunk's Fermat runs
+    chunks = [(c0, min(BATCH, n - c0)) for c0 in range(0, n, BATCH)]
+    c_index = torch.zeros(n, dtype=torch.int32, device="cuda")
+    c_count = torch.zeros(64 * len(chunks), dtype=torch.int32, device="cuda")
+
+    def pipeline_chunked():
+        for k, (c0, cn) in enumerate(chunks):
+            cb = pgx.Bucket(a.cand.data_ptr() + 4 * c0, a.ld, a.limbs, cn, a.max_bits,
+                            a.verdict.data_ptr() + 4 * c0, a.res.data_ptr() + 4 * c0, a.ld)
+            fermat.test_persistent(cb, pgx.MODE_FERMAT, residues=True, tuning=t,
+                                   stream=stream.cuda_stream)
+            idx = c_index.data_ptr() + 4 * c0
+            cnt = c_count.data_ptr() + 4 * 64 * k
+            fermat.compact(a.verdict.data_ptr() + 4 * c0, cn, pgx.FLAG_FERMAT, idx, cnt,
+                           stream.cuda_stream)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            for run, st, vv in ((r3, s3, v3), (r5, s5, v5)):
+                st.wait_event(ev)
+                mb = pgx.Bucket(a.cand.data_ptr() + 4 * c0, a.ld, a.limbs, cn, a.max_bits,
+                                vv.data_ptr() + 4 * c0, 0, a.ld)
+                run.test_indexed(mb, idx, cnt, pgx.MODE_STRONG,
+                                 tuning=pgx.Tuning(8, 4, 64, mr_lanes[0]), stream=st.cuda_stream)
+        for st in (s3, s5):
+            ev = torch.cuda.Event()
+            ev.record(st)
+            stream.wait_event(ev)
+
     pipeline()
     ms_fermat = time_ms(stream, fermat_only, reps)
     by_lanes = {}
@@ -672,6 +703,13 @@ def bench_large(stream, peak: float, reps: int) -> dict:
         pipeline()
         by_lanes[lanes] = time_ms(stream, pipeline, reps)
     mr_lanes[0] = min(by_lanes, key=by_lanes.get)
+    v3.fill_(-1)
+    v5.fill_(-1)
+    pipeline_chunked()
+    ms_chunked = time_ms(stream, pipeline_chunked, reps)
+    torch.cuda.synchronize()
+    f3c = v3.cpu().numpy().view(np.uint32).copy()
+    f5c = v5.cpu().numpy().view(np.uint32).copy()
     v3.fill_(-1)
     v5.fill_(-1)
     ms_all = time_ms(stream, pipeline, reps)
@@ -686,7 +724,8 @@ def bench_large(stream, peak: float, reps: int) -> dict:
     want = oracle.verdicts(L[s_sample], oracle.MODE_STRONG, (3, 5))
     ok = bool(np.array_equal(v[:1024], ov) and k == len(surv) and
               np.array_equal((f3[s_sample] == 1).astype(np.uint32) +
-                             ((f3[s_sample] == 1) & (f5[s_sample] == 1)).astype(np.uint32), want))
+                             ((f3[s_sample] == 1) & (f5[s_sample] == 1)).astype(np.uint32), want)
+              and np.array_equal(f3c[surv], f3[surv]) and np.array_equal(f5c[surv], f5[surv]))
     bits = median_bits(L)
     for r in (fermat, r3, r5):
         r.close()
@@ -701,6 +740,9 @@ def bench_large(stream, peak: float, reps: int) -> dict:
         "tuned": t.as_dict(),
         "fermat_ms": ms_fermat, "fermat_tests_per_s": n / (ms_fermat * 1e-3),
         "pipeline_ms": ms_all, "pipeline_tests_per_s": n / (ms_all * 1e-3),
+        "pipeline_chunked_ms": ms_chunked,
+        "pipeline_chunked_tests_per_s": n / (ms_chunked * 1e-3),
+        "pipeline_chunked_over_fermat": ms_chunked / ms_fermat,
         "mr_lanes_per_candidate": mr_lanes[0],
         "pipeline_ms_by_mr_lanes": {str(k): v for k, v in by_lanes.items()},
         "survivors": k, "prp_all_bases": int(len(prp)),
