@@ -1035,7 +1035,7 @@ def main() -> int:
                          "candidate (0,0,4 = never); cu%% > 0 runs those classes on that share of "
                          "the CUs and the narrower classes on the rest; `slots` iterations of a "
                          "class may be in flight at once")
-    ap.add_argument("--plain-classes", default="",
+    ap.add_argument("--plain-classes", default="1024,1260,2048",
                     help="comma-separated width classes (1024,1260,2048) to time through the "
                          "plain kernel alone, one entry per class in configs")
     ap.add_argument("--copy-probe", action="store_true",
