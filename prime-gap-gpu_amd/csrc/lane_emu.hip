@@ -11,8 +11,10 @@
 
 namespace {
 
-// mode bit selecting the folded squaring with shifted column sums
+// mode bits selecting the folded squaring with shifted column sums, and the
+// folded squaring that skips a zero top limb
 constexpr uint32_t kColumnShift = 0x100u;
+constexpr uint32_t kTop0 = 0x200u;
 
 template <int S>
 uint32_t one_candidate(const uint32_t* x, uint32_t limbs, uint32_t mode, const pgx::Bases& bases,
@@ -27,6 +29,8 @@ uint32_t one_candidate(const uint32_t* x, uint32_t limbs, uint32_t mode, const p
     auto word = [&](int j) { return x[j]; };
     if (mode == (PGX_MODE_FERMAT | kColumnShift))  // the squaring with column sums shifted
       v = pgx::prp_candidate<S, true, false>(w, word, PGX_MODE_FERMAT, bases, rw);
+    else if (mode & kTop0)  // the squaring that skips a zero top limb (base 2)
+      v = pgx::prp_candidate<S, true, true, true>(w, word, mode & ~kTop0, bases, rw);
     else if (mode == PGX_MODE_FERMAT || (bases.count == 1 && bases.a[0] == 2u))  // as launched
       v = pgx::prp_candidate<S, true>(w, word, mode, bases, rw);
     else
@@ -60,8 +64,9 @@ extern "C" {
 // (radix limbs; 0 = the smallest built one that holds every candidate).
 // Candidate-major limbs in, verdict words (and residues, candidate-major, if
 // res != NULL) out.  mode PGX_MODE_FERMAT | 0x100 runs the base-2 test with
-// the other form of the folded squaring.  Returns the S used, or 0 if none is
-// available.
+// the other form of the folded squaring; a base-2 mode | 0x200 the squaring
+// that skips a zero top limb (candidates of at most RB(S-1) - 1 bits).
+// Returns the S used, or 0 if none is available.
 uint32_t pgx_lane_emu(const uint32_t* limbs_le, size_t n, uint32_t limbs, uint32_t S,
                       uint32_t mode, const uint32_t* bases, uint32_t nbases, uint32_t* verdict,
                       uint32_t* res) {

@@ -46,6 +46,8 @@ struct Radix {
   static constexpr uint32_t MASK = (1u << RB) - 1u;
   // largest bit length a candidate may have: R = 2^(RB*S) >= 16 N
   static constexpr int MAX_BITS = RB * S - 4;
+  // largest bit length for the squaring with a zero top limb (x < 2N < 2^(RB(S-1)))
+  static constexpr int TOP0_MAX_BITS = RB * (S - 1) - 1;
   // 32-bit words spanned by S radix limbs
   static constexpr int W32 = (RB * S + 31) / 32;
 };
@@ -202,6 +204,53 @@ PGX_FN void mont_sqr_dbl(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv
     }
     x[S - 1] = (uint32_t)carry;
   }
+}
+
+// The folded squaring (operands shifted) when the top limb of x and of N is
+// zero: N < 2^(RB (S-1) - 1), so x < 2N fits in S-1 limbs, while R = 2^(RB S)
+// keeps its S digits (the folded doubling needs R > 8N; for N of RB(S-1) - 2
+// or RB(S-1) - 1 bits, S-1 digits fall short by a bit or two).  Every product
+// with limb S-1 of x or N is zero and is skipped: V(V+1)/2 + S V multiply-adds
+// instead of S(S+1)/2 + S^2 (V = S-1), one column and two operand shifts fewer.
+// The caller guarantees the bounds (powm_verdict runs it only in wavefronts
+// whose candidates all fit).
+template <int S>
+PGX_FN void mont_sqr_dbl_top0(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t ninv,
+                              uint32_t bit) {
+  constexpr int RB = Radix<S>::RB;
+  constexpr uint32_t MASK = Radix<S>::MASK;
+  constexpr int V = S - 1;  // value limbs
+  static_assert(S >= 2, "one value limb at least");
+  static_assert(RB == 28 || S <= 21, "column bound of the folded doubling");
+  uint32_t xs[V], xd[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    xs[i] = x[i] << bit;
+    xd[i] = xs[i] << 1;
+  }
+  uint32_t m[S];
+  uint64_t carry = 0;
+  // columns 0 .. S+V-2: x^2 fills 0 .. 2V-2, m*N fills 0 .. S+V-2
+#pragma unroll
+  for (int k = 0; k < S + V - 1; ++k) {
+    const int i0 = (k < V) ? 0 : k - V + 1;  // the other index stays below V
+    uint64_t t = carry;
+#pragma unroll
+    for (int i = i0; i < k - i; ++i) t += mul64(xd[i], x[k - i]);
+    if ((k & 1) == 0 && (k >> 1) < V) t += mul64(xs[k >> 1], x[k >> 1]);
+    const int i1 = (k < S) ? k : S;
+#pragma unroll
+    for (int i = i0; i < i1; ++i) t += mul64(m[i], n[k - i]);
+    if (k < S) {
+      m[k] = ((uint32_t)t * ninv) & MASK;
+      t += mul64(m[k], n[0]);  // low RB bits become zero
+    } else {
+      x[k - S] = (uint32_t)t & MASK;
+    }
+    carry = t >> RB;
+  }
+  x[V - 1] = (uint32_t)carry;
+  x[S - 1] = 0u;
 }
 
 // x <- x * y / R mod N (x < 4N, y < 4N with x*y < 16N^2; result < 2N).

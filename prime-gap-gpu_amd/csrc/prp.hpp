@@ -45,11 +45,14 @@ struct Bases {
 
 // Base 2: multiply by the base = doubling, folded into the squaring when the
 // radix leaves room for it (RB = 28, or RB = 29 with S <= 21).  kPreShift
-// selects the folded squaring's form (mont_sqr_dbl).
-template <int S, bool kPreShift = true>
+// selects the folded squaring's form (mont_sqr_dbl); kTop0 the folded squaring
+// that skips a zero top limb (mont_sqr_dbl_top0, candidates of at most
+// Radix<S>::TOP0_MAX_BITS bits).
+template <int S, bool kPreShift = true, bool kTop0_ = false>
 struct MulBy2 {
   static constexpr bool kFolded = (Radix<S>::RB == 28) || (S <= 21);
   static constexpr bool kShiftOperands = kPreShift;
+  static constexpr bool kTop0 = kTop0_ && kFolded && S >= 2;
   PGX_FN void operator()(uint32_t (&x)[S], const uint32_t (&)[S], uint32_t, uint32_t bit) const {
     dbl_lazy<S>(x, bit);
   }
@@ -61,6 +64,7 @@ template <int S>
 struct MulBySmall {
   static constexpr bool kFolded = false;
   static constexpr bool kShiftOperands = true;
+  static constexpr bool kTop0 = false;
   uint32_t a;
   double inv_n;
   int L;
@@ -69,31 +73,18 @@ struct MulBySmall {
   }
 };
 
-// Exponentiation + strong-test tail.  x enters as Mont(a) (the leading bit of
-// N-1); one = Mont(1) = R mod N, computed once by the caller (the tail
-// compares against it: recomputing it where p == t would repeat it for every
-// distinct t among a wavefront's lanes).  `word(j)` returns 32-bit word j of
-// N; it is read once per 32 exponent bits so the exponent never needs
-// dynamically indexed registers.  res gets a^(N-1) mod N (canonical); returns
-// FLAG_* bits.
-template <int S, class WordFn, class Step>
-PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&one)[S], const uint32_t (&n)[S],
-                             uint32_t ninv, int L, int t, WordFn word, const Step& step,
-                             uint32_t (&res)[S]) {
+// The exponent loop over bits Lw-2 .. 0 of N-1 (x enters as Mont(a), or
+// Mont(1) for lanes shorter than the wavefront's longest N, Lw) with the
+// strong-test checks at bits t .. 1.  kSkipTop: the squaring that skips a
+// zero top limb (every N of the wavefront has at most
+// Radix<S>::TOP0_MAX_BITS bits).  Returns 1 when a check passed.
+template <bool kSkipTop, int S, class WordFn, class Step>
+PGX_FN uint32_t exponent_bits(uint32_t (&x)[S], const uint32_t (&one)[S],
+                              const uint32_t (&n)[S], uint32_t ninv, int Lw, int t, int tw,
+                              WordFn word, const Step& step) {
   // a word, not a bool: a bool set in a divergent block is kept as a lane
   // mask in scalar registers, merged on every pass of the loop
   uint32_t sprp = 0;
-  if (t == L - 1) {  // N - 1 = 2^t: y_0 = a
-    canon<S>(x, n);
-    sprp = (equal<S>(x, one) || is_neg<S>(x, one, n)) ? 1u : 0u;
-  }
-  // wavefront-uniform bounds: the longest N and the largest t of the wavefront
-  const int Lw = (int)wave_max<12>((uint32_t)L);
-  const int tw = (int)wave_max<12>((uint32_t)t);
-  if (L != Lw) {  // bits Lw-2 .. L of N-1 are zero: start from Mont(1)
-#pragma unroll
-    for (int j = 0; j < S; ++j) x[j] = one[j];
-  }
   const int top = (Lw - 2) >> 5;  // word of the first bit
   uint32_t w_next = word(top);
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -114,7 +105,9 @@ PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&one)[S], const u
     int b = (wi == top) ? ((Lw - 2) & 31) : 31;
     do {
       const uint32_t bit = (w >> b) & 1u;
-      if constexpr (Step::kFolded) {
+      if constexpr (kSkipTop) {
+        mont_sqr_dbl_top0<S>(x, n, ninv, bit);
+      } else if constexpr (Step::kFolded) {
         mont_sqr_dbl<S, Step::kShiftOperands>(x, n, ninv, bit);
       } else {
         mont_sqr<S>(x, n, ninv);
@@ -128,6 +121,42 @@ PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&one)[S], const u
         }
       }
     } while (--b >= 0);
+  }
+  return sprp;
+}
+
+// Exponentiation + strong-test tail.  x enters as Mont(a) (the leading bit of
+// N-1); one = Mont(1) = R mod N, computed once by the caller (the tail
+// compares against it: recomputing it where p == t would repeat it for every
+// distinct t among a wavefront's lanes).  `word(j)` returns 32-bit word j of
+// N; it is read once per 32 exponent bits so the exponent never needs
+// dynamically indexed registers.  res gets a^(N-1) mod N (canonical); returns
+// FLAG_* bits.  A Step with kTop0 runs the top-limb-zero loop in wavefronts
+// whose candidates all fit it and the folded loop in the others, so the
+// results never depend on which loop ran.
+template <int S, class WordFn, class Step>
+PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&one)[S], const uint32_t (&n)[S],
+                             uint32_t ninv, int L, int t, WordFn word, const Step& step,
+                             uint32_t (&res)[S]) {
+  uint32_t sprp = 0;
+  if (t == L - 1) {  // N - 1 = 2^t: y_0 = a
+    canon<S>(x, n);
+    sprp = (equal<S>(x, one) || is_neg<S>(x, one, n)) ? 1u : 0u;
+  }
+  // wavefront-uniform bounds: the longest N and the largest t of the wavefront
+  const int Lw = (int)wave_max<12>((uint32_t)L);
+  const int tw = (int)wave_max<12>((uint32_t)t);
+  if (L != Lw) {  // bits Lw-2 .. L of N-1 are zero: start from Mont(1)
+#pragma unroll
+    for (int j = 0; j < S; ++j) x[j] = one[j];
+  }
+  if constexpr (Step::kTop0) {
+    if (Lw <= Radix<S>::TOP0_MAX_BITS)
+      sprp |= exponent_bits<true>(x, one, n, ninv, Lw, t, tw, word, step);
+    else
+      sprp |= exponent_bits<false>(x, one, n, ninv, Lw, t, tw, word, step);
+  } else {
+    sprp |= exponent_bits<false>(x, one, n, ninv, Lw, t, tw, word, step);
   }
 #pragma unroll
   for (int j = 0; j < S; ++j) res[j] = x[j];
@@ -145,7 +174,7 @@ PGX_FN uint32_t powm_verdict(uint32_t (&x)[S], const uint32_t (&one)[S], const u
   return f;
 }
 
-template <int S, bool kPreShift, class WordFn>
+template <int S, bool kPreShift, bool kTop0 = false, class WordFn>
 PGX_FN uint32_t test_base2(const uint32_t (&n)[S], uint32_t ninv, int L, int t, WordFn word,
                            uint32_t (&res)[S]) {
   uint32_t one[S], x[S];
@@ -153,7 +182,7 @@ PGX_FN uint32_t test_base2(const uint32_t (&n)[S], uint32_t ninv, int L, int t, 
 #pragma unroll
   for (int j = 0; j < S; ++j) x[j] = one[j];
   dbl_full<S>(x, n);  // Mont(2)
-  return powm_verdict<S>(x, one, n, ninv, L, t, word, MulBy2<S, kPreShift>{}, res);
+  return powm_verdict<S>(x, one, n, ninv, L, t, word, MulBy2<S, kPreShift, kTop0>{}, res);
 }
 
 // Base 2 <= a < 2^8 (the extra Miller-Rabin bases).
@@ -177,10 +206,12 @@ PGX_FN uint32_t test_base(uint32_t a, const uint32_t (&n)[S], uint32_t ninv, int
 // of `mode` (PGX_VERDICT_INVALID if N is even, <= 1 or wider than the kernel)
 // and writes the first base's a^(N-1) mod N as W32 words into res_w.
 // kBase2Only: the base list is {2} (or the mode is FERMAT, which tests base 2
-// whatever the list).
-template <int S, bool kBase2Only, bool kPreShift = true, class WordFn>
+// whatever the list).  kTop0 (base 2 only): the squaring that skips a zero top
+// limb in wavefronts whose candidates fit it (same results).
+template <int S, bool kBase2Only, bool kPreShift = true, bool kTop0 = false, class WordFn>
 PGX_FN uint32_t prp_candidate(const uint32_t (&w)[Radix<S>::W32], WordFn word, uint32_t mode,
                               const Bases& bases, uint32_t (&res_w)[Radix<S>::W32]) {
+  static_assert(!kTop0 || kBase2Only, "the top-limb-zero squaring serves base 2");
   constexpr int W = Radix<S>::W32;
   const int L = bit_length_words<W>(w);
   uint32_t r[S];
@@ -195,7 +226,7 @@ PGX_FN uint32_t prp_candidate(const uint32_t (&w)[Radix<S>::W32], WordFn word, u
     if (kBase2Only) {
       // base 2 alone: FERMAT mode returns the flags; the reference and strong
       // modes with the base list {2} return the count of bases passed
-      const uint32_t f = test_base2<S, kPreShift>(n, ninv, L, t, word, r);
+      const uint32_t f = test_base2<S, kPreShift, kTop0>(n, ninv, L, t, word, r);
       const uint32_t need = (mode == PGX_MODE_STRONG) ? FLAG_SPRP : FLAG_QUIRK;
       v = (mode == PGX_MODE_FERMAT) ? f : ((f & need) ? 1u : 0u);
     } else {

@@ -42,8 +42,9 @@ __device__ __forceinline__ void test_column(const PrpArgs& a, uint32_t p) {
   uint32_t rw[W];
   uint32_t v = PGX_VERDICT_INVALID;
   if (spill == 0) {
-    v = prp_candidate<S, kBase2Only, kPreShift>(w, [&](int j) { return c[j * ld]; }, a.mode,
-                                                a.bases, rw);
+    // base 2 alone: the top-limb-zero squaring where a wavefront's candidates fit it
+    v = prp_candidate<S, kBase2Only, kPreShift, kBase2Only>(
+        w, [&](int j) { return c[j * ld]; }, a.mode, a.bases, rw);
   } else {
 #pragma unroll
     for (int j = 0; j < W; ++j) rw[j] = 0;

@@ -177,6 +177,38 @@ def test_lane_routine_every_width_gap(emu):
         assert np.array_equal(v, O.verdicts(L, O.MODE_STRONG, (2, 3, 5))), S
 
 
+def test_lane_routine_top_limb_zero_squaring(emu):
+    """The base-2 squaring that skips a zero top limb (mode bit 0x200) holds
+    for N of at most 29(S-1) - 1 bits; a lane with a longer N runs the
+    ordinary folded loop.  Random odd N from 60 bits below that bound up to
+    the kernel's top (both loops, and the bound itself on either side),
+    residues and verdicts against GMP in every base-2 mode.  S = 26 has no
+    folded squaring (the column bound) and must give the same results too."""
+    import random
+    rng = random.Random(11)
+    for S in (2, 3, 5, 12, 17, 26):
+        bound = 29 * (S - 1) - 1
+        top = 29 * S - 4
+        lens = range(max(3, bound - 60), top + 1)
+        vals = [rng.getrandbits(b) | (1 << (b - 1)) | 1 for b in lens for _ in range(3)]
+        vals += [(1 << bound) - 1, (1 << bound) + 1, (1 << (bound - 1)) + 1]
+        L = pgx.to_limbs(vals, (29 * S + 31) // 32)
+        v, r = emu(L, S, pgx.MODE_FERMAT | 0x200)
+        ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+        assert np.array_equal(v, ov) and np.array_equal(r, orr), S
+        for mode, omode in ((pgx.MODE_REFERENCE, O.MODE_REFERENCE),
+                            (pgx.MODE_STRONG, O.MODE_STRONG)):
+            v, _ = emu(L, S, mode | 0x200, (2,))
+            assert np.array_equal(v, O.verdicts(L, omode, (2,))), (S, mode)
+    # the 462-bit search candidates (17 limbs: bound 463 bits) and small N
+    m = pgx.gen_multipliers(0x5EED0006, 10**7, 10**7 + 10**4, 2310, 2)
+    L, _, _ = pgx.gen_sieved(337, 2310, m, 200000, 300, 16)
+    for arr in (L, pgx.to_limbs(list(range(3, 1 << 12, 2)), 16)):
+        v, r = emu(arr, 17, pgx.MODE_FERMAT | 0x200)
+        ov, orr = O.verdicts(arr, O.MODE_FERMAT, residues=True)
+        assert np.array_equal(v, ov) and np.array_equal(r, orr)
+
+
 def test_lane_routine_rejects_what_does_not_fit(emu):
     # even, one, and a value too wide for one radix-2^29 limb (R = 2^29 >= 16N
     # allows 25-bit N)
