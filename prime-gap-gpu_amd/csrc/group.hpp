@@ -45,9 +45,12 @@ constexpr uint32_t MASK = (1u << RB) - 1u;
 
 constexpr int qp(int a, int b, int c, int d) { return a | (b << 2) | (c << 4) | (d << 6); }
 
+// Every control used with dpp() reads a valid lane for every lane (quad
+// permutations, row broadcasts and rotations), so no old value is needed and
+// none is materialized.
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
 }
 
 // DPP row shifts (lanes of a row of 16; lanes shifted in from outside the
@@ -56,9 +59,10 @@ constexpr int kRowShl1 = 0x101, kRowShl4 = 0x104, kRowShr1 = 0x111, kRowShr4 = 0
 constexpr int kRowRor4 = 0x124, kRowRor8 = 0x128, kWaveShl1 = 0x130, kWaveShr1 = 0x138;
 constexpr int kRowBcast15 = 0x142, kRowNewBcast0 = 0x150;
 
+// Shifts: lanes with no source lane read 0 (bound_ctrl).
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_row(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
 }
 
 // In groups of 8: the value of the lane 4 above (lanes 0-3 of the group) or 4
@@ -276,13 +280,14 @@ __device__ __forceinline__ void settle_u(uint32_t (&r)[Q], const uint64_t (&t)[Q
 // argument so the owner lane of a_I and its slot are constants.  Every lane
 // splits its lowest slot (position I + lane*Q) at bit RB: the carry stays in
 // the lane, added to the next slot, and only the low RB bits move to the top
-// slot of the lane below (one 32-bit DPP move; lane 0's low bits are zero by
-// the choice of m and leave the window).  top_keep is 0 in the top lane of the
-// group, which has nothing above it, and all ones elsewhere.
+// slot of the lane below (one 32-bit DPP move of the slot's low word, masked
+// by the receiver; lane 0's low bits are zero by the choice of m and leave the
+// window).  low_keep is 0 in the top lane of the group, which has nothing
+// above it, and MASK elsewhere.
 template <int Q, int G, int I>
 __device__ __forceinline__ void scan_step(uint64_t (&acc)[Q], const uint32_t (&a)[Q],
                                           const uint32_t (&b)[Q], const uint32_t (&n)[Q],
-                                          uint32_t ninv, uint32_t top_keep) {
+                                          uint32_t ninv, uint32_t low_keep) {
   static_assert(Q > 1, "the carry of the lowest slot stays in the lane");
   const uint32_t ai = bcast<G, I / Q>(a[I % Q]);
 #pragma unroll
@@ -290,9 +295,8 @@ __device__ __forceinline__ void scan_step(uint64_t (&acc)[Q], const uint32_t (&a
   const uint32_t m = bcast<G, 0>(((uint32_t)acc[0] * ninv) & MASK);
 #pragma unroll
   for (int s = 0; s < Q; ++s) acc[s] += (uint64_t)m * n[s];
-  const uint32_t lo = (uint32_t)acc[0] & MASK;
   acc[1] += acc[0] >> RB;
-  const uint32_t nx = from_next<G>(lo) & top_keep;
+  const uint32_t nx = from_next<G>((uint32_t)acc[0]) & low_keep;
 #pragma unroll
   for (int s = 0; s + 1 < Q; ++s) acc[s] = acc[s + 1];
   acc[Q - 1] = nx;
@@ -302,8 +306,8 @@ template <int Q, int G, int... Is>
 __device__ __forceinline__ void scan_all(std::integer_sequence<int, Is...>, uint64_t (&acc)[Q],
                                          const uint32_t (&a)[Q], const uint32_t (&b)[Q],
                                          const uint32_t (&n)[Q], uint32_t ninv,
-                                         uint32_t top_keep) {
-  (scan_step<Q, G, Is>(acc, a, b, n, ninv, top_keep), ...);
+                                         uint32_t low_keep) {
+  (scan_step<Q, G, Is>(acc, a, b, n, ninv, low_keep), ...);
 }
 
 // r <- a * b / R mod N (a, b < 4N with normalized limbs; r < 2N).  r may
@@ -315,8 +319,8 @@ __device__ __forceinline__ void mont_mul(uint32_t (&r)[Q], const uint32_t (&a)[Q
   uint64_t acc[Q];
 #pragma unroll
   for (int s = 0; s < Q; ++s) acc[s] = 0;
-  const uint32_t top_keep = (lane_id<G>() == G - 1) ? 0u : ~0u;
-  scan_all<Q, G>(std::make_integer_sequence<int, Q * G>{}, acc, a, b, n, ninv, top_keep);
+  const uint32_t low_keep = (lane_id<G>() == G - 1) ? 0u : MASK;
+  scan_all<Q, G>(std::make_integer_sequence<int, Q * G>{}, acc, a, b, n, ninv, low_keep);
   settle_u<Q, G>(r, acc);
 }
 
