@@ -1,0 +1,362 @@
+#!/usr/bin/env python3
+"""Generates the exponent loop of the base-2 one-lane kernels as inline assembly.
+
+For a kernel with S radix-2^RB limbs the generated function runs the bits
+b_hi .. b_lo of one 32-bit exponent word: per bit, x <- 2^bit * x^2 / R mod N
+(R = 2^(RB*S), values lazy in [0, 2N)), exactly the operation of mont_sqr_dbl /
+mont_sqr_dbl_top0 in mont.hpp, so the residues do not depend on which path ran.
+
+Why assembly.  At one wavefront per SIMD (a 65,536-candidate launch on 1,024
+SIMDs) every instruction takes an issue slot and gfx950 has two costly
+patterns that the compiler does not avoid:
+  * a scalar instruction issued within a few vector instructions of a
+    v_mad_u64_u32 (which writes an SGPR carry-out) waits ~110 cycles; the
+    compiled loop computes its bit index and loop condition right after the
+    last multiply-add of the squaring;
+  * a VOP2 (_e32) instruction with two VGPR sources whose result feeds
+    multiply-adds costs ~200 cycles per squaring; the VOP3 form costs nothing
+    (the compiled squaring shifts its operands by the exponent bit that way).
+The generated loop puts its scalar work after the operand shifts at the top of
+each iteration (no multiply-add for at least 8 vector instructions before it),
+computes the loop condition there so the closing branch does not wait, and
+uses VOP3 encodings wherever two VGPRs are read.  Dependencies between vector
+instructions cost nothing on this target (measured), so the column order of
+the C++ routine is kept and no latency scheduling is needed.
+
+Two forms of the folded doubling:
+  * "rdbl": the reduction divides by R / 2^bit (the last reduction digit has
+    RB - bit bits and the upper columns are shifted by bit as they join the
+    carry: v_lshl_add_u64 with the bit as shift); only xd = 2x is formed.
+  * "fold": the operands are shifted (xs = x << bit, xd = xs << 1), as
+    mont_sqr_dbl does.
+rdbl needs fewer instructions; it is used where its column sums provably stay
+below 2^64 (checked exactly below for all-ones limbs), fold elsewhere, and a
+width where neither fits keeps the C++ loop, as do widths above MAX_S (the
+register allocator's time grows steeply with the operand count of one
+statement: 25 s at 36 limbs, 45 s at 46, over 15 minutes at 64).
+
+The script also interprets every instruction list it emits on random inputs
+(Python integers) and checks the result against x^2 2^bit R^-1 mod N, the lazy
+bound and 64-bit overflow, so a wrong list fails at build time.
+
+Usage: gen_sqr.py OUT.inc S1,S2,...
+"""
+import random
+import sys
+
+M64 = (1 << 64) - 1
+MAX_S = 48
+
+
+def radix(S):
+    return 29 if S <= 31 else 28
+
+
+def columns(S, V):
+    """Products of each column of x^2 + m*N, as (kind, i, j) triples."""
+    cols = []
+    for k in range(S + V - 1):
+        i0 = 0 if k < V else k - V + 1
+        prods = []
+        for i in range(i0, k):
+            if i < k - i:
+                prods.append(('cross', i, k - i))
+        if k % 2 == 0 and k // 2 < V:
+            prods.append(('square', k // 2, k // 2))
+        i1 = k if k < S else S
+        for i in range(i0, i1):
+            if k - i < V:
+                prods.append(('red', i, k - i))
+        cols.append(prods)
+    return cols
+
+
+def bound_ok(S, top0, form):
+    """Exact worst-case column sums (all limbs 2^RB - 1, bit = 1) stay < 2^64."""
+    RB = radix(S)
+    V = S - 1 if top0 else S
+    L = (1 << RB) - 1
+    xd = (2 * L) if form == 'rdbl' else (4 * L)   # cross operand
+    xs = L if form == 'rdbl' else 2 * L          # square operand
+    carry = 0
+    for k, prods in enumerate(columns(S, V)):
+        s = 0
+        for kind, i, j in prods:
+            s += xd * L if kind == 'cross' else (xs * L if kind == 'square' else L * L)
+        if k < S:
+            t = s + carry + L * L       # + m_k n_0
+            sh = RB - 1 if (form == 'rdbl' and k == S - 1) else RB
+        else:
+            t = (2 * s if form == 'rdbl' else s) + carry
+            sh = RB
+        if t > M64:
+            return False
+        carry = t >> sh
+    return True
+
+
+class Gen:
+    """Instruction list of one squaring (columns in order) over named values.
+
+    Names: x{i} (limbs, updated in place), xd{i}/xs{i} (shifted operands),
+    n{i}, m{i} (reduction digits), acc (the unshifted sum of an upper column
+    of rdbl), c0/c1 (carries), T0/T1 (fixed 64-bit pairs whose low words are
+    addressable: column sums accumulate there).
+    """
+
+    def __init__(self, S, top0, form):
+        self.S, self.top0, self.form = S, top0, form
+        self.RB = radix(S)
+        self.V = S - 1 if top0 else S
+        self.ops = []
+
+    def emit(self, kind, dst, srcs, extra=None):
+        self.ops.append((kind, dst, srcs, extra))
+
+    def build(self):
+        S, V, RB, form = self.S, self.V, self.RB, self.form
+        cols = columns(S, V)
+        carry = None
+        for k, prods in enumerate(cols):
+            T = 'T0' if k % 2 == 0 else 'T1'
+            # The carry of the previous column seeds the first multiply-add of
+            # the column (a dependent chain costs nothing here), except for the
+            # upper columns of rdbl, whose sum is shifted by the bit first.
+            shifted = (k >= S and form == 'rdbl')
+            acc = 'acc' if shifted else T
+            seed = None if (k == 0 or shifted) else carry
+            first = True
+            for kind, i, j in prods:
+                if kind == 'cross':
+                    a, b = f'xd{i}', f'x{j}'
+                elif kind == 'square':
+                    a, b = (f'x{i}' if form == 'rdbl' else f'xs{i}'), f'x{j}'
+                else:
+                    a, b = f'm{i}', f'n{j}'
+                if first and seed is None:
+                    self.emit('mad', acc, [a, b], 'init')
+                else:
+                    self.emit('mad', acc, [a, b, seed if first else acc])
+                first = False
+            if k < S:
+                last = (k == S - 1 and form == 'rdbl')
+                self.emit('mul', f'm{k}', [T + '.lo', 'ninv'])
+                self.emit('and', f'm{k}', [f'm{k}'], 'mlast' if last else 'mask')
+                self.emit('mad', T, [f'm{k}', 'n0', T])
+                carry = f'c{k % 2}'
+                self.emit('shr64', carry, [T], 'shl' if last else RB)
+            else:
+                if shifted:
+                    self.emit('lshl_add', T, [acc, carry], 'bit')
+                self.emit('and', f'x{k - S}', [T + '.lo'], 'mask')
+                if k == S + V - 2:
+                    T2 = 'T1' if T == 'T0' else 'T0'
+                    self.emit('shr64', T2, [T], RB)
+                    self.emit('mov', f'x{V - 1}', [T2 + '.lo'])
+                else:
+                    carry = f'c{k % 2}'
+                    self.emit('shr64', carry, [T], RB)
+        return self.ops
+
+    def prologue(self):
+        """Per-bit operand preparation (before the scalar loop control)."""
+        V = self.V
+        ops = []
+        if self.form == 'rdbl':
+            ops.append(('sub_rb', 'shl', ['bit'], None))
+            ops.append(('bfe_mask', 'mlast', ['shl'], None))
+            for i in range(V):
+                ops.append(('shl1', f'xd{i}', [f'x{i}'], None))
+        else:
+            for i in range(V):
+                ops.append(('shlbit', f'xs{i}', [f'x{i}'], None))
+                ops.append(('shl1', f'xd{i}', [f'xs{i}'], None))
+        return ops
+
+
+def interpret(S, top0, form, ops, x, n, ninv, bit):
+    RB = radix(S)
+    V = S - 1 if top0 else S
+    mask = (1 << RB) - 1
+    r = {'ninv': ninv, 'bit': bit}
+    for i in range(V):
+        r[f'x{i}'] = x[i]
+        r[f'n{i}'] = n[i]
+
+    def rd(v):
+        return r[v[:-3]] & 0xffffffff if v.endswith('.lo') else r[v]
+    for kind, dst, srcs, extra in ops:
+        s = [rd(v) for v in srcs]
+        if kind == 'mad':
+            assert s[0] < 2 ** 32 and s[1] < 2 ** 32
+            v = s[0] * s[1] + (0 if extra == 'init' else s[2])
+            assert v <= M64, 'multiply-add overflow'
+        elif kind == 'lshl_add':
+            v = (s[0] << (bit if extra == 'bit' else extra)) + s[1]
+            assert v <= M64, 'add overflow'
+        elif kind == 'shr64':
+            v = s[0] >> ((RB - bit) if extra == 'shl' else extra)
+        elif kind == 'mul':
+            v = (s[0] * s[1]) & 0xffffffff
+        elif kind == 'and':
+            v = s[0] & ((mask >> bit) if extra == 'mlast' else mask)
+        elif kind == 'mov':
+            v = s[0] & 0xffffffff
+        elif kind == 'sub_rb':
+            v = RB - s[0]
+        elif kind == 'bfe_mask':
+            v = (1 << s[0]) - 1
+        elif kind == 'shl1':
+            v = s[0] << 1
+            assert v < 2 ** 32
+        elif kind == 'shlbit':
+            v = s[0] << bit
+            assert v < 2 ** 32
+        else:
+            raise ValueError(kind)
+        r[dst] = v
+    return [r[f'x{i}'] for i in range(V)]
+
+
+def verify(S, top0, form, trials=200, seed=1):
+    g = Gen(S, top0, form)
+    ops = g.prologue() + g.build()
+    RB, V = g.RB, g.V
+    mask = (1 << RB) - 1
+    rng = random.Random(seed * 1000 + S)
+    R = 1 << (RB * S)
+    maxbits = RB * (S - 1) - 1 if top0 else RB * S - 4
+    for t in range(trials):
+        bits = rng.randint(max(2, maxbits - 40), maxbits)
+        N = rng.getrandbits(bits) | (1 << (bits - 1)) | 1
+        x = 2 * N - 1 - t if t < 4 else rng.randrange(0, 2 * N)
+        bit = t & 1 if t < 4 else rng.randint(0, 1)
+        ninv = (-pow(N, -1, 1 << 32)) & 0xffffffff
+        xl = [(x >> (RB * i)) & mask for i in range(V)]
+        nl = [(N >> (RB * i)) & mask for i in range(V)]
+        got = interpret(S, top0, form, ops, xl, nl, ninv, bit)
+        assert all(v <= mask for v in got), f'S={S}: limb range'
+        g_val = sum(v << (RB * i) for i, v in enumerate(got))
+        assert g_val % N == (x * x * (1 << bit) * pow(R, -1, N)) % N, f'S={S}: wrong value'
+        assert g_val < 2 * N, f'S={S}: lazy bound'
+    return ops
+
+
+def asm_text(S, top0, form, ops):
+    RB = radix(S)
+    maskhex = '0x%x' % ((1 << RB) - 1)
+    fixed = {'T0': 'v[0:1]', 'T1': 'v[2:3]', 'T0.lo': 'v0', 'T1.lo': 'v2'}
+
+    def o(v):
+        return fixed.get(v, '%[' + v + ']')
+    lines = []
+    for kind, dst, srcs, extra in ops:
+        s = [o(v) for v in srcs]
+        d = o(dst)
+        if kind == 'mad':
+            c = '0' if extra == 'init' else s[2]
+            lines.append(f'v_mad_u64_u32 {d}, %[sc], {s[0]}, {s[1]}, {c}')
+        elif kind == 'lshl_add':
+            sh = '%[bit]' if extra == 'bit' else str(extra)
+            lines.append(f'v_lshl_add_u64 {d}, {s[0]}, {sh}, {s[1]}')
+        elif kind == 'shr64':
+            sh = '%[shl]' if extra == 'shl' else str(extra)
+            lines.append(f'v_lshrrev_b64 {d}, {sh}, {s[0]}')
+        elif kind == 'mul':
+            lines.append(f'v_mul_lo_u32 {d}, {s[0]}, {s[1]}')
+        elif kind == 'and':
+            if extra == 'mlast':   # two VGPR sources: VOP3 encoding
+                lines.append(f'v_and_b32_e64 {d}, %[mlast], {s[0]}')
+            else:
+                lines.append(f'v_and_b32_e32 {d}, {maskhex}, {s[0]}')
+        elif kind == 'mov':
+            lines.append(f'v_mov_b32_e32 {d}, {s[0]}')
+        elif kind == 'sub_rb':
+            lines.append(f'v_sub_u32_e64 {d}, {RB}, {s[0]}')
+        elif kind == 'bfe_mask':
+            lines.append(f'v_bfe_u32 {d}, -1, 0, {s[0]}')
+        elif kind == 'shl1':
+            lines.append(f'v_lshlrev_b32_e32 {d}, 1, {s[0]}')
+        elif kind == 'shlbit':   # shift amount in a VGPR: VOP3 encoding
+            lines.append(f'v_lshlrev_b32_e64 {d}, %[bit], {s[0]}')
+    return lines
+
+
+def function_text(S, top0):
+    if S > MAX_S or S < 4:  # below 4 limbs (candidates under 113 bits) the C++ loop stays
+        return None
+    form = 'rdbl' if bound_ok(S, top0, 'rdbl') else ('fold' if bound_ok(S, top0, 'fold') else None)
+    if form is None:
+        return None
+    g = Gen(S, top0, form)
+    pro, body = g.prologue(), g.build()
+    verify(S, top0, form)
+    V = g.V
+    nvalu_before_salu = 1 + len(pro)
+    pad = ['v_nop'] * max(0, 8 - nvalu_before_salu - 4)
+    lines = (['s_mov_b32 %[sb], %[bhi]', '.p2align 6', '1:',
+              'v_bfe_u32 %[bit], %[w], %[sb], 1'] + asm_text(S, top0, form, pro) + pad +
+             ['s_sub_u32 %[sb], %[sb], 1', 's_cmp_ge_i32 %[sb], %[blo]'] +
+             asm_text(S, top0, form, body) + ['s_cbranch_scc1 1b'])
+    names32, names64 = [], []
+    for kind, dst, srcs, extra in pro + body:
+        for v in [dst] + list(srcs):
+            b = v.split('.')[0]
+            if b.startswith(('T',)) or b.startswith('x') and b[1:].isdigit() or b.startswith('n') and b[1:].isdigit() or b in ('ninv', 'bit'):
+                continue
+            if b.startswith(('acc', 'c')) and b not in names64:
+                names64.append(b)
+            elif not b.startswith(('acc', 'c')) and b not in names32:
+                names32.append(b)
+    outs = [f'[x{i}] "+v"(x[{i}])' for i in range(V)]
+    outs += [f'[{v}] "=&v"({v})' for v in names32]
+    outs += [f'[{v}] "=&v"({v})' for v in names64]
+    outs += ['[bit] "=&v"(bit)', '[sc] "=&s"(sc)', '[sb] "=&s"(sb)']
+    ins = [f'[n{i}] "v"(n[{i}])' for i in range(V)] + ['[ninv] "v"(ninv)', '[w] "v"(w)', '[bhi] "s"(b_hi)', '[blo] "s"(b_lo)']
+    body_txt = '\n'.join(f'      "{l}\\n"' for l in lines)
+    return f"""// S = {S}, {'top limb zero' if top0 else 'all limbs'}, form {form}: {len(body)} instructions per squaring
+template <>
+struct AsmSquarings<{S}, {'true' if top0 else 'false'}> {{
+  static constexpr bool kAvailable = true;
+  __device__ __forceinline__ static void run(uint32_t (&x)[{S}], const uint32_t (&n)[{S}], uint32_t ninv,
+                                             uint32_t w, int b_hi, int b_lo) {{
+    uint32_t {', '.join(names32)};
+    uint64_t {', '.join(names64)};
+    uint32_t bit, sb;
+    uint64_t sc;
+    asm volatile(
+{body_txt}
+        : {', '.join(outs)}
+        : {', '.join(ins)}
+        : "v0", "v1", "v2", "v3", "scc");
+    (void)sc;
+    (void)bit;
+    (void)sb;
+  }}
+}};
+"""
+
+
+def main():
+    out, widths = sys.argv[1], [int(s) for s in sys.argv[2].split(',') if s]
+    parts = ['// Generated by gen_sqr.py; do not edit.', '#pragma once', '']
+    for S in widths:
+        for top0 in (False, True):
+            if top0 and S < 2:
+                continue
+            t = function_text(S, top0)
+            if t:
+                parts.append(t)
+    text = '\n'.join(parts) + '\n'
+    try:
+        with open(out) as f:
+            if f.read() == text:  # unchanged: keep the timestamp (no rebuild)
+                return
+    except OSError:
+        pass
+    with open(out, 'w') as f:
+        f.write(text)
+
+
+if __name__ == '__main__':
+    main()

@@ -43,6 +43,21 @@ struct Bases {
   uint32_t a[PGX_MAX_BASES];
 };
 
+// Base-2 squarings of one exponent word as generated assembly (gen_sqr.py):
+// run(x, n, ninv, w, b_hi, b_lo) does, for b = b_hi down to b_lo,
+// x <- 2^((w >> b) & 1) x^2 / R mod N, the same operation as mont_sqr_dbl /
+// mont_sqr_dbl_top0 (kTop0).  A translation unit built with
+// PGX_ASM_SQUARINGS_INC includes the generated specializations; a width
+// without one keeps the C++ loop.
+template <int S, bool kTop0>
+struct AsmSquarings {
+  static constexpr bool kAvailable = false;
+  __device__ static void run(uint32_t (&)[S], const uint32_t (&)[S], uint32_t, uint32_t, int, int) {}
+};
+#if defined(PGX_ASM_SQUARINGS_INC)
+#include PGX_ASM_SQUARINGS_INC
+#endif
+
 // Base 2: multiply by the base = doubling, folded into the squaring when the
 // radix leaves room for it (RB = 28, or RB = 29 with S <= 21).  kPreShift
 // selects the folded squaring's form (mont_sqr_dbl); kTop0 the folded squaring
@@ -87,6 +102,32 @@ PGX_FN uint32_t exponent_bits(uint32_t (&x)[S], const uint32_t (&one)[S],
   uint32_t sprp = 0;
   const int top = (Lw - 2) >> 5;  // word of the first bit
   uint32_t w_next = word(top);
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (AsmSquarings<S, kSkipTop>::kAvailable && Step::kFolded && Step::kShiftOperands) {
+    // Generated loop: the bits above tw (no strong-test check) of a word in
+    // one call, then the checked bits one at a time; one call site, so one
+    // copy of the squaring.
+    for (int wi = top; wi >= 0; --wi) {
+      const uint32_t w = (wi == 0) ? (w_next & ~1u) : w_next;
+      if (wi > 0) w_next = word(wi - 1);
+      int b = (wi == top) ? ((Lw - 2) & 31) : 31;
+      while (b >= 0) {
+        const int free_lo = max(tw + 1 - 32 * wi, 0);  // lowest bit of the word above tw
+        const int lo = (b >= free_lo) ? free_lo : b;
+        AsmSquarings<S, kSkipTop>::run(x, n, ninv, w, b, lo);
+        const int p = 32 * wi + lo;
+        if (p <= tw) {
+          if (p <= t && p >= 1) {  // x = Mont(y_{t-p})
+            canon<S>(x, n);
+            if (is_neg<S>(x, one, n) || (p == t && equal<S>(x, one))) sprp = 1u;
+          }
+        }
+        b = lo - 1;
+      }
+    }
+    return sprp;
+  }
+#endif
 #if defined(__HIP_DEVICE_COMPILE__)
   // Place the exponent loop just after a 256-byte boundary (the padding runs
   // once).  With the loop body identical instruction for instruction, its
