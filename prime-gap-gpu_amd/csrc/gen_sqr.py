@@ -27,13 +27,15 @@ Two forms of the folded doubling:
   * "rdbl": the reduction divides by R / 2^bit (the last reduction digit has
     RB - bit bits and the upper columns are shifted by bit as they join the
     carry: v_lshl_add_u64 with the bit as shift); only xd = 2x is formed.
-  * "fold": the operands are shifted (xs = x << bit, xd = xs << 1), as
+  * "fold": the operands are shifted (xd = x << (bit + 1) for the cross
+    products, x << bit = xd >> 1 formed just before each square product), as
     mont_sqr_dbl does.
 rdbl needs fewer instructions; it is used where its column sums provably stay
 below 2^64 (checked exactly below for all-ones limbs), fold elsewhere, and a
 width where neither fits keeps the C++ loop, as do widths above MAX_S (the
 register allocator's time grows steeply with the operand count of one
-statement: 25 s at 36 limbs, 45 s at 46, over 15 minutes at 64).
+statement: 25 s at 36 limbs, 45 s at 46, over 15 minutes at 64; temporaries
+in fixed registers instead of operands took longer still).
 
 The script also interprets every instruction list it emits on random inputs
 (Python integers) and checks the result against x^2 2^bit R^-1 mod N, the lazy
@@ -130,7 +132,11 @@ class Gen:
                 if kind == 'cross':
                     a, b = f'xd{i}', f'x{j}'
                 elif kind == 'square':
-                    a, b = (f'x{i}' if form == 'rdbl' else f'xs{i}'), f'x{j}'
+                    if form == 'rdbl':
+                        a, b = f'x{i}', f'x{j}'
+                    else:  # x_i << bit, formed from xd_i just before its use
+                        self.emit('shr1', 'xsq', [f'xd{i}'])
+                        a, b = 'xsq', f'x{j}'
                 else:
                     a, b = f'm{i}', f'n{j}'
                 if first and seed is None:
@@ -168,9 +174,9 @@ class Gen:
             for i in range(V):
                 ops.append(('shl1', f'xd{i}', [f'x{i}'], None))
         else:
+            ops.append(('add1', 'bit1', ['bit'], None))
             for i in range(V):
-                ops.append(('shlbit', f'xs{i}', [f'x{i}'], None))
-                ops.append(('shl1', f'xd{i}', [f'xs{i}'], None))
+                ops.append(('shlbit1', f'xd{i}', [f'x{i}'], None))
         return ops
 
 
@@ -209,9 +215,13 @@ def interpret(S, top0, form, ops, x, n, ninv, bit):
         elif kind == 'shl1':
             v = s[0] << 1
             assert v < 2 ** 32
-        elif kind == 'shlbit':
-            v = s[0] << bit
+        elif kind == 'shlbit1':
+            v = s[0] << s[1] if len(s) > 1 else s[0] << (bit + 1)
             assert v < 2 ** 32
+        elif kind == 'add1':
+            v = s[0] + 1
+        elif kind == 'shr1':
+            v = s[0] >> 1
         else:
             raise ValueError(kind)
         r[dst] = v
@@ -242,10 +252,11 @@ def verify(S, top0, form, trials=200, seed=1):
     return ops
 
 
-def asm_text(S, top0, form, ops):
+def asm_text(S, top0, form, ops, regs=None):
     RB = radix(S)
     maskhex = '0x%x' % ((1 << RB) - 1)
     fixed = {'T0': 'v[0:1]', 'T1': 'v[2:3]', 'T0.lo': 'v0', 'T1.lo': 'v2'}
+    fixed.update(regs or {})
 
     def o(v):
         return fixed.get(v, '%[' + v + ']')
@@ -260,13 +271,13 @@ def asm_text(S, top0, form, ops):
             sh = '%[bit]' if extra == 'bit' else str(extra)
             lines.append(f'v_lshl_add_u64 {d}, {s[0]}, {sh}, {s[1]}')
         elif kind == 'shr64':
-            sh = '%[shl]' if extra == 'shl' else str(extra)
+            sh = o('shl') if extra == 'shl' else str(extra)
             lines.append(f'v_lshrrev_b64 {d}, {sh}, {s[0]}')
         elif kind == 'mul':
             lines.append(f'v_mul_lo_u32 {d}, {s[0]}, {s[1]}')
         elif kind == 'and':
             if extra == 'mlast':   # two VGPR sources: VOP3 encoding
-                lines.append(f'v_and_b32_e64 {d}, %[mlast], {s[0]}')
+                lines.append(f'v_and_b32_e64 {d}, {o("mlast")}, {s[0]}')
             else:
                 lines.append(f'v_and_b32_e32 {d}, {maskhex}, {s[0]}')
         elif kind == 'mov':
@@ -277,8 +288,12 @@ def asm_text(S, top0, form, ops):
             lines.append(f'v_bfe_u32 {d}, -1, 0, {s[0]}')
         elif kind == 'shl1':
             lines.append(f'v_lshlrev_b32_e32 {d}, 1, {s[0]}')
-        elif kind == 'shlbit':   # shift amount in a VGPR: VOP3 encoding
-            lines.append(f'v_lshlrev_b32_e64 {d}, %[bit], {s[0]}')
+        elif kind == 'shlbit1':   # shift amount in a VGPR: VOP3 encoding
+            lines.append(f'v_lshlrev_b32_e64 {d}, {o("bit1")}, {s[0]}')
+        elif kind == 'add1':
+            lines.append(f'v_add_u32_e64 {d}, {s[0]}, 1')
+        elif kind == 'shr1':
+            lines.append(f'v_lshrrev_b32_e32 {d}, 1, {s[0]}')
     return lines
 
 
@@ -294,10 +309,6 @@ def function_text(S, top0):
     V = g.V
     nvalu_before_salu = 1 + len(pro)
     pad = ['v_nop'] * max(0, 8 - nvalu_before_salu - 4)
-    lines = (['s_mov_b32 %[sb], %[bhi]', '.p2align 6', '1:',
-              'v_bfe_u32 %[bit], %[w], %[sb], 1'] + asm_text(S, top0, form, pro) + pad +
-             ['s_sub_u32 %[sb], %[sb], 1', 's_cmp_ge_i32 %[sb], %[blo]'] +
-             asm_text(S, top0, form, body) + ['s_cbranch_scc1 1b'])
     names32, names64 = [], []
     for kind, dst, srcs, extra in pro + body:
         for v in [dst] + list(srcs):
@@ -308,27 +319,32 @@ def function_text(S, top0):
                 names64.append(b)
             elif not b.startswith(('acc', 'c')) and b not in names32:
                 names32.append(b)
+    regs, clob = {}, ['v0', 'v1', 'v2', 'v3']
+    lines = (['s_mov_b32 %[sb], %[bhi]', '.p2align 6', '1:',
+              'v_bfe_u32 %[bit], %[w], %[sb], 1'] + asm_text(S, top0, form, pro, regs) + pad +
+             ['s_sub_u32 %[sb], %[sb], 1', 's_cmp_ge_i32 %[sb], %[blo]'] +
+             asm_text(S, top0, form, body, regs) + ['s_cbranch_scc1 1b'])
     outs = [f'[x{i}] "+v"(x[{i}])' for i in range(V)]
     outs += [f'[{v}] "=&v"({v})' for v in names32]
     outs += [f'[{v}] "=&v"({v})' for v in names64]
     outs += ['[bit] "=&v"(bit)', '[sc] "=&s"(sc)', '[sb] "=&s"(sb)']
     ins = [f'[n{i}] "v"(n[{i}])' for i in range(V)] + ['[ninv] "v"(ninv)', '[w] "v"(w)', '[bhi] "s"(b_hi)', '[blo] "s"(b_lo)']
     body_txt = '\n'.join(f'      "{l}\\n"' for l in lines)
+    decl = ''.join(f'    {t} {", ".join(n)};\n' for t, n in (('uint32_t', names32), ('uint64_t', names64)) if n)
+    clob_txt = ', '.join(f'"{c}"' for c in clob + ['scc'])
     return f"""// S = {S}, {'top limb zero' if top0 else 'all limbs'}, form {form}: {len(body)} instructions per squaring
 template <>
 struct AsmSquarings<{S}, {'true' if top0 else 'false'}> {{
   static constexpr bool kAvailable = true;
   __device__ __forceinline__ static void run(uint32_t (&x)[{S}], const uint32_t (&n)[{S}], uint32_t ninv,
                                              uint32_t w, int b_hi, int b_lo) {{
-    uint32_t {', '.join(names32)};
-    uint64_t {', '.join(names64)};
-    uint32_t bit, sb;
+{decl}    uint32_t bit, sb;
     uint64_t sc;
     asm volatile(
 {body_txt}
         : {', '.join(outs)}
         : {', '.join(ins)}
-        : "v0", "v1", "v2", "v3", "scc");
+        : {clob_txt});
     (void)sc;
     (void)bit;
     (void)sb;
