@@ -31,8 +31,8 @@ Two forms of the folded doubling:
     products, x << bit = xd >> 1 formed just before each square product), as
     mont_sqr_dbl does.
 rdbl needs fewer instructions; it is used where its column sums provably stay
-below 2^64 (checked exactly below for all-ones limbs), fold elsewhere, and a
-width where neither fits keeps the C++ loop, as do widths above MAX_S (the
+below 2^64 (checked exactly below for all-ones limbs), then fold, then
+sqrshift; a width where none fits keeps the C++ loop, as do widths above MAX_S (the
 register allocator's time grows steeply with the operand count of one
 statement: 25 s at 36 limbs, 45 s at 46, over 15 minutes at 64; temporaries
 in fixed registers instead of operands took longer still).
@@ -78,8 +78,10 @@ def bound_ok(S, top0, form):
     RB = radix(S)
     V = S - 1 if top0 else S
     L = (1 << RB) - 1
-    xd = (2 * L) if form == 'rdbl' else (4 * L)   # cross operand
-    xs = L if form == 'rdbl' else 2 * L          # square operand
+    if form == 'sqrshift' and top0:
+        return False  # the doubled value needs the top limb
+    xd = (4 * L) if form == 'fold' else (2 * L)   # cross operand
+    xs = (2 * L) if form == 'fold' else L         # square operand
     carry = 0
     for k, prods in enumerate(columns(S, V)):
         s = 0
@@ -132,7 +134,7 @@ class Gen:
                 if kind == 'cross':
                     a, b = f'xd{i}', f'x{j}'
                 elif kind == 'square':
-                    if form == 'rdbl':
+                    if form in ('rdbl', 'sqrshift'):
                         a, b = f'x{i}', f'x{j}'
                     else:  # x_i << bit, formed from xd_i just before its use
                         self.emit('shr1', 'xsq', [f'xd{i}'])
@@ -162,15 +164,25 @@ class Gen:
                 else:
                     carry = f'c{k % 2}'
                     self.emit('shr64', carry, [T], RB)
+        if form == 'sqrshift':
+            # x <- 2^bit x (lazy: < 4N), limbs from the top down so that each
+            # reads its lower neighbour before that one is shifted
+            for j in range(V - 1, 0, -1):
+                self.emit('shr_var', 'tsh', [f'x{j - 1}', 'shl'])
+                self.emit('lshl_or', f'x{j}', [f'x{j}', 'bit', 'tsh'])
+                self.emit('and', f'x{j}', [f'x{j}'], 'mask')
+            self.emit('shl_var', 'x0', ['x0', 'bit'])
+            self.emit('and', 'x0', ['x0'], 'mask')
         return self.ops
 
     def prologue(self):
         """Per-bit operand preparation (before the scalar loop control)."""
         V = self.V
         ops = []
-        if self.form == 'rdbl':
+        if self.form in ('rdbl', 'sqrshift'):
             ops.append(('sub_rb', 'shl', ['bit'], None))
-            ops.append(('bfe_mask', 'mlast', ['shl'], None))
+            if self.form == 'rdbl':
+                ops.append(('bfe_mask', 'mlast', ['shl'], None))
             for i in range(V):
                 ops.append(('shl1', f'xd{i}', [f'x{i}'], None))
         else:
@@ -220,6 +232,12 @@ def interpret(S, top0, form, ops, x, n, ninv, bit):
             assert v < 2 ** 32
         elif kind == 'add1':
             v = s[0] + 1
+        elif kind == 'shr_var':
+            v = s[0] >> s[1]
+        elif kind == 'lshl_or':
+            v = ((s[0] << s[1]) | s[2]) & 0xffffffff
+        elif kind == 'shl_var':
+            v = (s[0] << s[1]) & 0xffffffff
         elif kind == 'shr1':
             v = s[0] >> 1
         else:
@@ -239,7 +257,8 @@ def verify(S, top0, form, trials=200, seed=1):
     for t in range(trials):
         bits = rng.randint(max(2, maxbits - 40), maxbits)
         N = rng.getrandbits(bits) | (1 << (bits - 1)) | 1
-        x = 2 * N - 1 - t if t < 4 else rng.randrange(0, 2 * N)
+        lazy = 4 if form == 'sqrshift' else 2   # values in [0, lazy N)
+        x = lazy * N - 1 - t if t < 4 else rng.randrange(0, lazy * N)
         bit = t & 1 if t < 4 else rng.randint(0, 1)
         ninv = (-pow(N, -1, 1 << 32)) & 0xffffffff
         xl = [(x >> (RB * i)) & mask for i in range(V)]
@@ -248,7 +267,7 @@ def verify(S, top0, form, trials=200, seed=1):
         assert all(v <= mask for v in got), f'S={S}: limb range'
         g_val = sum(v << (RB * i) for i, v in enumerate(got))
         assert g_val % N == (x * x * (1 << bit) * pow(R, -1, N)) % N, f'S={S}: wrong value'
-        assert g_val < 2 * N, f'S={S}: lazy bound'
+        assert g_val < lazy * N, f'S={S}: lazy bound'
     return ops
 
 
@@ -292,6 +311,12 @@ def asm_text(S, top0, form, ops, regs=None):
             lines.append(f'v_lshlrev_b32_e64 {d}, {o("bit1")}, {s[0]}')
         elif kind == 'add1':
             lines.append(f'v_add_u32_e64 {d}, {s[0]}, 1')
+        elif kind == 'shr_var':
+            lines.append(f'v_lshrrev_b32_e64 {d}, {s[1]}, {s[0]}')
+        elif kind == 'lshl_or':
+            lines.append(f'v_lshl_or_b32 {d}, {s[0]}, {s[1]}, {s[2]}')
+        elif kind == 'shl_var':
+            lines.append(f'v_lshlrev_b32_e64 {d}, {s[1]}, {s[0]}')
         elif kind == 'shr1':
             lines.append(f'v_lshrrev_b32_e32 {d}, 1, {s[0]}')
     return lines
@@ -300,7 +325,7 @@ def asm_text(S, top0, form, ops, regs=None):
 def function_text(S, top0):
     if S > MAX_S or S < 4:  # below 4 limbs (candidates under 113 bits) the C++ loop stays
         return None
-    form = 'rdbl' if bound_ok(S, top0, 'rdbl') else ('fold' if bound_ok(S, top0, 'fold') else None)
+    form = next((f for f in ('rdbl', 'fold', 'sqrshift') if bound_ok(S, top0, f)), None)
     if form is None:
         return None
     g = Gen(S, top0, form)

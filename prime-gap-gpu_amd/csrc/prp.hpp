@@ -46,7 +46,8 @@ struct Bases {
 // Base-2 squarings of one exponent word as generated assembly (gen_sqr.py):
 // run(x, n, ninv, w, b_hi, b_lo) does, for b = b_hi down to b_lo,
 // x <- 2^((w >> b) & 1) x^2 / R mod N, the same operation as mont_sqr_dbl /
-// mont_sqr_dbl_top0 (kTop0).  A translation unit built with
+// mont_sqr_dbl_top0 (kTop0), or mont_sqr + dbl_lazy where the doubling is not
+// folded (radix 2^29 above 21 limbs).  A translation unit built with
 // PGX_ASM_SQUARINGS_INC includes the generated specializations; a width
 // without one keeps the C++ loop.
 template <int S, bool kTop0>
@@ -65,6 +66,7 @@ struct AsmSquarings {
 // Radix<S>::TOP0_MAX_BITS bits).
 template <int S, bool kPreShift = true, bool kTop0_ = false>
 struct MulBy2 {
+  static constexpr bool kBase2 = true;
   static constexpr bool kFolded = (Radix<S>::RB == 28) || (S <= 21);
   static constexpr bool kShiftOperands = kPreShift;
   static constexpr bool kTop0 = kTop0_ && kFolded && S >= 2;
@@ -77,6 +79,7 @@ struct MulBy2 {
 // (mul_small, about 11 S instructions instead of a Montgomery product).
 template <int S>
 struct MulBySmall {
+  static constexpr bool kBase2 = false;
   static constexpr bool kFolded = false;
   static constexpr bool kShiftOperands = true;
   static constexpr bool kTop0 = false;
@@ -103,7 +106,7 @@ PGX_FN uint32_t exponent_bits(uint32_t (&x)[S], const uint32_t (&one)[S],
   const int top = (Lw - 2) >> 5;  // word of the first bit
   uint32_t w_next = word(top);
 #if defined(__HIP_DEVICE_COMPILE__)
-  if constexpr (AsmSquarings<S, kSkipTop>::kAvailable && Step::kFolded && Step::kShiftOperands) {
+  if constexpr (AsmSquarings<S, kSkipTop>::kAvailable && Step::kBase2 && Step::kShiftOperands) {
     // Generated loop: the bits above tw (no strong-test check) of a word in
     // one call, then the checked bits one at a time; one call site, so one
     // copy of the squaring.

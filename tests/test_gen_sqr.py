@@ -21,9 +21,7 @@ WIDTHS = list(range(1, 49)) + [56, 64, 74]
 
 
 def _form(S, top0):
-    if gen.bound_ok(S, top0, "rdbl"):
-        return "rdbl"
-    return "fold" if gen.bound_ok(S, top0, "fold") else None
+    return next((f for f in ("rdbl", "fold", "sqrshift") if gen.bound_ok(S, top0, f)), None)
 
 
 @pytest.mark.parametrize("S", [S for S in WIDTHS if 4 <= S <= gen.MAX_S])
@@ -37,12 +35,15 @@ def test_every_generated_width_interprets_correctly(S):
 
 def test_forms_match_the_column_bounds_of_the_cpp_loop():
     # the C++ loop folds the doubling for RB = 28 and for RB = 29 up to 21
-    # limbs (static_assert in mont.hpp); every such width has a generated form
+    # limbs (static_assert in mont.hpp); there a folded form is generated
     for S in WIDTHS:
         if 4 <= S <= gen.MAX_S and (S >= 32 or S <= 21):
-            assert _form(S, False) is not None, S
-    # radix 2^29 beyond 21 limbs fits neither form with all limbs in use
-    assert _form(25, False) is None
+            assert _form(S, False) in ("rdbl", "fold"), S
+    # radix 2^29 beyond 21 limbs fits neither folded form with all limbs in
+    # use: squaring then doubling, as the C++ loop's mont_sqr + dbl_lazy
+    for S in range(22, 32):
+        assert not gen.bound_ok(S, False, "fold") and _form(S, False) == "sqrshift", S
+        assert not gen.bound_ok(S, True, "sqrshift")
 
 
 def test_bound_check_is_exact_at_the_edge():
