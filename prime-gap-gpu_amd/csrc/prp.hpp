@@ -55,6 +55,15 @@ struct AsmSquarings {
   static constexpr bool kAvailable = false;
   __device__ static void run(uint32_t (&)[S], const uint32_t (&)[S], uint32_t, uint32_t, int, int) {}
 };
+// The same for a small Miller-Rabin base a: run(..., a - 1, (1 - 2^-40) / N)
+// does x <- x^2 / R mod N, then x <- a x mod N where bit b is set (mont_sqr
+// then mul_small, with the quotient estimated from all limbs).
+template <int S>
+struct AsmSmallBase {
+  static constexpr bool kAvailable = false;
+  __device__ static void run(uint32_t (&)[S], const uint32_t (&)[S], uint32_t, uint32_t, int, int,
+                             uint32_t, double) {}
+};
 #if defined(PGX_ASM_SQUARINGS_INC)
 #include PGX_ASM_SQUARINGS_INC
 #endif
@@ -106,10 +115,18 @@ PGX_FN uint32_t exponent_bits(uint32_t (&x)[S], const uint32_t (&one)[S],
   const int top = (Lw - 2) >> 5;  // word of the first bit
   uint32_t w_next = word(top);
 #if defined(__HIP_DEVICE_COMPILE__)
-  if constexpr (AsmSquarings<S, kSkipTop>::kAvailable && Step::kBase2 && Step::kShiftOperands) {
+  constexpr bool kAsm2 = AsmSquarings<S, kSkipTop>::kAvailable && Step::kBase2 && Step::kShiftOperands;
+  constexpr bool kAsmA = AsmSmallBase<S>::kAvailable && !Step::kBase2 && !kSkipTop;
+  if constexpr (kAsm2 || kAsmA) {
     // Generated loop: the bits above tw (no strong-test check) of a word in
     // one call, then the checked bits one at a time; one call site, so one
     // copy of the squaring.
+    uint32_t am1 = 0;
+    double invn = 0.0;
+    if constexpr (kAsmA) {
+      am1 = step.a - 1u;
+      invn = ldexp(step.inv_n, -step.L) * (1.0 - 0x1p-40);  // exact scaling, one rounding
+    }
     for (int wi = top; wi >= 0; --wi) {
       const uint32_t w = (wi == 0) ? (w_next & ~1u) : w_next;
       if (wi > 0) w_next = word(wi - 1);
@@ -117,7 +134,10 @@ PGX_FN uint32_t exponent_bits(uint32_t (&x)[S], const uint32_t (&one)[S],
       while (b >= 0) {
         const int free_lo = max(tw + 1 - 32 * wi, 0);  // lowest bit of the word above tw
         const int lo = (b >= free_lo) ? free_lo : b;
-        AsmSquarings<S, kSkipTop>::run(x, n, ninv, w, b, lo);
+        if constexpr (kAsm2)
+          AsmSquarings<S, kSkipTop>::run(x, n, ninv, w, b, lo);
+        else
+          AsmSmallBase<S>::run(x, n, ninv, w, b, lo, am1, invn);
         const int p = 32 * wi + lo;
         if (p <= tw) {
           if (p <= t && p >= 1) {  // x = Mont(y_{t-p})

@@ -33,6 +33,18 @@ def test_every_generated_width_interprets_correctly(S):
         gen.verify(S, top0, form, trials=60, seed=7)
 
 
+@pytest.mark.parametrize("S", [4, 5, 9, 13, 17, 20, 25, 31, 32, 36])
+def test_small_base_form_interprets_correctly(S):
+    # squaring, then a' x mod N with the quotient estimated in double precision
+    # (interpreted with correctly rounded fused multiply-adds and products)
+    assert gen.bound_ok(S, False, "mr") and not gen.bound_ok(S, True, "mr")
+    gen.verify(S, False, "mr", trials=60, seed=11)
+
+
+def test_small_base_form_stops_where_one_double_cannot_hold_the_limbs():
+    assert gen.bound_ok(36, False, "mr") and not gen.bound_ok(37, False, "mr")
+
+
 def test_forms_match_the_column_bounds_of_the_cpp_loop():
     # the C++ loop folds the doubling for RB = 28 and for RB = 29 up to 21
     # limbs (static_assert in mont.hpp); there a folded form is generated
@@ -83,20 +95,36 @@ def test_interpreter_catches_a_dropped_product():
     assert sum(v << (RB * i) for i, v in enumerate(bad)) % N != want
 
 
-def test_generated_text_has_no_two_vgpr_vop2_and_one_loop():
-    text = gen.function_text(17, True)
+def _size(line):
+    """Encoded bytes of an instruction the generator emits (gfx950)."""
+    op = line.split(" ", 1)[0]
+    if op.startswith("s_"):
+        return 4
+    if op.endswith("_e32"):
+        srcs = line.split(",")[1:]
+        lit = any(x.strip().startswith("0x") for x in srcs)
+        return 8 if lit else 4
+    return 8  # VOP3 / VOP3b
+
+
+@pytest.mark.parametrize("S,top0,small", [(17, True, False), (17, False, False), (20, False, False),
+                                          (25, False, False), (36, True, False), (17, False, True)])
+def test_generated_loop_is_aligned_with_scalar_work_away_from_the_mads(S, top0, small):
+    text = gen.function_text(S, top0, small)
     body = [l.strip().strip('"').replace("\\n", "") for l in text.splitlines() if l.strip().startswith('"')]
     assert sum(1 for l in body if l.startswith("s_cbranch_scc1 1b")) == 1
-    for l in body:
-        op = l.split(" ", 1)[0]
-        if op.endswith("_e32"):
-            srcs = [s.strip() for s in l.split(",")[1:]]
-            vgpr_like = [s for s in srcs if s.startswith("%[") or s.startswith("v")]
-            assert len(vgpr_like) <= 1, l
+    assert body[body.index("1:") - 1] == ".p2align 6"
+    # every 8-byte instruction of the loop starts on an 8-byte boundary
+    off = 0
+    for l in body[body.index("1:") + 1:]:
+        n = _size(l)
+        if n == 8:
+            assert off % 8 == 0, l
+        off += n
     # the scalar loop control comes after at least 8 vector instructions that
-    # follow the previous iteration's last multiply-add
+    # follow the previous iteration's last SGPR-writing multiply-add
     start = body.index("1:")
     salu = next(i for i in range(start, len(body)) if body[i].startswith("s_sub_u32"))
-    last_mad = max(i for i, l in enumerate(body) if l.startswith("v_mad_u64_u32"))
+    last_mad = max(i for i, l in enumerate(body) if l.startswith(("v_mad_u64_u32", "v_mad_i64_i32")))
     tail = len(body) - 1 - last_mad - 1  # vector instructions after the last mad, before the branch
     assert tail + (salu - start - 1) >= 8
