@@ -671,10 +671,13 @@ def bench_large(stream, peak: float, reps: int) -> dict:
             ev.record(st)
             stream.wait_event(ev)
 
-    # chunked: Fermat over 65536-candidate chunks on the main stream, each
-    # chunk's survivors compacted and tested with bases 3 and 5 on the side
-    # streams while the next chunk's Fermat runs
+    # chunked: Fermat over 65536-candidate chunks on the main stream (plain
+    # grids: a shape tuned on the whole bucket can leave a chunk with fewer
+    # work chunks than wavefronts), each chunk's survivors compacted and
+    # tested with bases 3 and 5 on the side streams while the next chunk's
+    # Fermat runs
     chunks = [(c0, min(BATCH, n - c0)) for c0 in range(0, n, BATCH)]
+    t_chunk = pgx.Tuning(0, 4, 64, 1)
     c_index = torch.zeros(n, dtype=torch.int32, device="cuda")
     c_count = torch.zeros(64 * len(chunks), dtype=torch.int32, device="cuda")
 
@@ -682,7 +685,7 @@ def bench_large(stream, peak: float, reps: int) -> dict:
         for k, (c0, cn) in enumerate(chunks):
             cb = pgx.Bucket(a.cand.data_ptr() + 4 * c0, a.ld, a.limbs, cn, a.max_bits,
                             a.verdict.data_ptr() + 4 * c0, a.res.data_ptr() + 4 * c0, a.ld)
-            fermat.test_persistent(cb, pgx.MODE_FERMAT, residues=True, tuning=t,
+            fermat.test_persistent(cb, pgx.MODE_FERMAT, residues=True, tuning=t_chunk,
                                    stream=stream.cuda_stream)
             idx = c_index.data_ptr() + 4 * c0
             cnt = c_count.data_ptr() + 4 * 64 * k

@@ -195,8 +195,9 @@ const char* pgx_last_error(const pgx_runner_t* h);
  * BITS (gpu_testing.cu:369-385).  Here candidates of any mix of sizes are
  * grouped into buckets (one kernel limb count each), every bucket runs as a
  * persistent kernel whose resident wavefronts take chunks of candidates from
- * a device counter, and a plan captures all buckets (parallel graph branches,
- * counters reset by a fill-kernel node) into one hipGraph launch. */
+ * a device counter, and a plan captures all buckets (counters reset by a
+ * fill-kernel node) into one hipGraph launch: buckets of at least one
+ * wavefront per SIMD one after another, smaller ones in parallel branches. */
 typedef struct {
   const uint32_t* d_cand; /* limb-interleaved candidates: limb j of column p at d_cand[j*ld + p] */
   uint32_t ld;            /* leading dimension, >= n */

@@ -216,9 +216,14 @@ int pgx_autotune(pgx_runner_t* h, const pgx_bucket_t* b, uint32_t flags, pgx_tun
       for (uint32_t ch : chunks) {
         if (rc != PGX_OK) break;
         if (w * bl > 32) continue;  // beyond the 32 wavefronts a CU can hold
-        // more wavefronts than chunks of work: some would idle, and where the
-        // busy ones land decides the time (the plain grid covers this case)
-        if ((uint64_t)h->n_cu * bl * w > ((uint64_t)sample.n + ch - 1) / ch) continue;
+        // fewer than two chunks per wavefront: alone on the GPU every
+        // wavefront takes its one chunk and the time is level, but next to
+        // other kernels (a plan's concurrent buckets) the wavefronts start at
+        // different times and a chunk is a large part of the whole, so the
+        // last one to start sets the time (2^19 987-bit candidates, 4 blocks of
+        // one wavefront per CU: 256-candidate chunks took 48.8 or 58.9 ms in a
+        // plan beside three other buckets, 64-candidate chunks 48.9-51.0)
+        if ((uint64_t)h->n_cu * bl * w * 2 > ((uint64_t)sample.n + ch - 1) / ch) continue;
         pgx_tuning_t t{bl, w, ch, 1};
         float shape_ms = 0;
         for (int rep = 0; rep < kTuneReps && rc == PGX_OK; ++rep) {
@@ -306,20 +311,30 @@ int pgx_plan_create(pgx_plan_t** out, pgx_runner_t* h, const pgx_bucket_t* bucke
     delete p;
     return fail(h, PGX_E_HIP, "plan resources: %s", hipGetErrorString(e));
   }
-  // capture: fork from streams[0], one branch per bucket, join back
+  // capture: a bucket of at least one wavefront per SIMD fills the GPU by
+  // itself and runs on the capture stream after the previous such bucket
+  // (side by side, their persistent wavefronts start at staggered times and
+  // a tuned shape loses its balance: 2^21 candidates in four buckets took
+  // 48.8-58.9 ms as four branches against 48.7-51.3 ms one after another);
+  // smaller buckets fork into branches of their own that fill the gaps, and
+  // every branch joins back before the end
   hipStream_t cap = p->streams[0];
+  const uint64_t fills = (uint64_t)h->n_cu * 4u * 64u;
   e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) e = hipEventRecord(p->events[0], cap);
   for (uint32_t k = 0; k < nbuckets && e == hipSuccess; ++k) {
-    hipStream_t s = p->streams[k + 1];
-    e = hipStreamWaitEvent(s, p->events[0], 0);
-    if (e == hipSuccess && buckets[k].n > 0) {
+    if (buckets[k].n == 0) continue;
+    const bool own_branch = buckets[k].n < fills;
+    hipStream_t s = own_branch ? p->streams[k + 1] : cap;
+    if (own_branch) e = hipStreamWaitEvent(s, p->events[0], 0);
+    if (e == hipSuccess) {
       pgx::PrpArgs a = args_of(h, &buckets[k], flags);
       e = launch_tuned(h, S[k], T[k], a, buckets[k].max_bits, p->d_counters + k, s);
     }
-    if (e == hipSuccess) e = hipEventRecord(p->events[k + 1], s);
-    if (e == hipSuccess) e = hipStreamWaitEvent(cap, p->events[k + 1], 0);
+    if (own_branch && e == hipSuccess) e = hipEventRecord(p->events[k + 1], s);
   }
+  for (uint32_t k = 0; k < nbuckets && e == hipSuccess; ++k)
+    if (buckets[k].n > 0 && buckets[k].n < fills) e = hipStreamWaitEvent(cap, p->events[k + 1], 0);
   hipError_t e_end = hipStreamEndCapture(cap, &p->graph);
   if (e == hipSuccess) e = e_end;
   if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
