@@ -406,22 +406,33 @@ int pgx_submit(pgx_runner_t* h, const uint32_t* limbs_le, size_t n, uint32_t fla
     return fail(h, PGX_E_CAPACITY, "tests(%zu) > n(%zu)", n, h->max_n);
   if ((flags & PGX_MODE_MASK) > PGX_MODE_FERMAT) return fail(h, PGX_E_ARG, "bad mode");
   const uint32_t S = h->limbs;
+  // Candidate-major through pinned staging, about 1 MiB at a time: each
+  // piece is checked (odd, 2 <= bits <= the widest kernel) and copied while
+  // it is in the host's cache, and its DMA runs while the next piece is
+  // checked.  d_res holds the rows until the transpose has read them (the
+  // test writes its residues there afterwards, in stream order).
+  const size_t piece = std::max<size_t>(1, (size_t)(1u << 20) / (4u * S));
   uint32_t max_bits = 0;
-  for (size_t i = 0; i < n; ++i) {
-    const uint32_t* x = limbs_le + i * S;
-    const uint32_t L = bit_length(x, S);
-    if (!(x[0] & 1u) || L < 2)
-      return fail(h, PGX_E_INPUT, "candidate %zu is even or <= 1", i);
-    if (L > PGX_MAX_BITS)
-      return fail(h, PGX_E_INPUT, "candidate %zu has %u bits, more than the kernels support", i, L);
-    max_bits = std::max(max_bits, L);
+  for (size_t c0 = 0; c0 < n; c0 += piece) {
+    const size_t cn = std::min(piece, n - c0);
+    const uint32_t* src = limbs_le + c0 * S;
+    for (size_t i = 0; i < cn; ++i) {
+      const uint32_t* x = src + i * S;
+      const uint32_t L = bit_length(x, S);
+      if (!(x[0] & 1u) || L < 2 || L > PGX_MAX_BITS) {
+        // earlier pieces may still be in flight from the staging buffer
+        (void)hipStreamSynchronize(b.stream);
+        if (!(x[0] & 1u) || L < 2)
+          return fail(h, PGX_E_INPUT, "candidate %zu is even or <= 1", c0 + i);
+        return fail(h, PGX_E_INPUT, "candidate %zu has %u bits, more than the kernels support",
+                    c0 + i, L);
+      }
+      max_bits = std::max(max_bits, L);
+    }
+    std::memcpy(b.h_cand + c0 * S, src, cn * S * 4);
+    PGX_HIP(h, hipMemcpyAsync(b.d_res + c0 * S, b.h_cand + c0 * S, cn * S * 4,
+                              hipMemcpyHostToDevice, b.stream));
   }
-  // Candidate-major through pinned staging; d_res holds the rows until the
-  // transpose has read them (the test writes its residues there afterwards,
-  // in stream order).
-  std::memcpy(b.h_cand, limbs_le, (size_t)S * n * 4);
-  PGX_HIP(h, hipMemcpyAsync(b.d_res, b.h_cand, (size_t)S * n * 4, hipMemcpyHostToDevice,
-                            b.stream));
   PGX_HIP(h, launch_transpose(true, b.d_res, b.d_cand, n, S, (uint32_t)n, b.stream));
   int rc = launch_batch(h, b, n, max_bits, flags);
   if (rc != PGX_OK) return rc;

@@ -166,6 +166,21 @@ def test_submit_wait_and_errors():
         assert list(run.run(pgx.to_limbs([13], 2), pgx.MODE_FERMAT) & 1) == [1]
 
 
+def test_rejection_past_the_first_staging_piece():
+    # the host checks and stages candidates about 1 MiB at a time, copying
+    # each piece to the device before the next is checked: a bad candidate in
+    # a later piece is still reported by index, and the runner stays usable
+    n = (1 << 20) // 8 + 5
+    with pgx.Runner(n, limbs=2) as run:
+        L = pgx.to_limbs([7] * n, 2)
+        L[n - 2, 0] = 8
+        with pytest.raises(pgx.PgxError) as e:
+            run.run(L, pgx.MODE_FERMAT)
+        assert e.value.code == pgx.PGX_E_INPUT and f"candidate {n - 2} " in str(e.value)
+        v = run.run(pgx.to_limbs([7] * (n - 1) + [9], 2), pgx.MODE_FERMAT)
+        assert (v[:-1] & 1).all() and not (v[-1] & 1)
+
+
 def test_device_resident_interleaved_layout():
     """pgx_test_device on torch-owned HBM buffers, limb-interleaved, ld > n."""
     import torch
