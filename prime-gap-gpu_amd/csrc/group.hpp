@@ -383,16 +383,7 @@ __device__ __forceinline__ void set_pow2(uint32_t (&v)[Q], int bit) {
   for (int s = 0; s < Q; ++s) v[s] = (s == q) ? (1u << rr) : 0u;
 }
 
-// one <- R mod N by doubling 2^(L-1) (< N).
-template <int Q, int G>
-__device__ __forceinline__ void r_mod_n(uint32_t (&one)[Q], const uint32_t (&n)[Q], int L) {
-  set_pow2<Q, G>(one, L - 1);
-  const int steps = RB * Q * G - (L - 1);
-  for (int c = 0; c < steps; ++c) {
-    dbl_lazy<Q, G>(one, 1u);
-    sub_if_ge<Q, G>(one, n);
-  }
-}
+
 
 // Sum of v_j 2^(28 j - L) over the group's limbs, as a double.
 template <int Q, int G>
@@ -404,8 +395,9 @@ __device__ __forceinline__ double scaled(const uint32_t (&v)[Q], int L) {
   return group_sum<G>(d);
 }
 
-// x <- a x mod N for an ordinary integer 2 <= a < 2^8 (x < 4N in, < 2N out);
-// the quotient estimate of mont.hpp's mul_small, summed over the group.
+// x <- a x mod N for an ordinary integer 1 <= a <= 2^16 (x < 4N in, < 2N
+// out); the quotient estimate of mont.hpp's mul_small, summed over the group
+// (a x / N < 2^18: the 2^-40 bias still moves it by less than 2^-21).
 template <int Q, int G>
 __device__ __forceinline__ void mul_small(uint32_t (&x)[Q], uint32_t a, const uint32_t (&n)[Q],
                                           double inv_n, int L) {
@@ -416,6 +408,17 @@ __device__ __forceinline__ void mul_small(uint32_t (&x)[Q], uint32_t a, const ui
   for (int s = 0; s < Q; ++s)
     t[s] = (int64_t)((uint64_t)x[s] * a) - (int64_t)((uint64_t)n[s] * q);
   settle<Q, G>(x, t);
+}
+
+// one <- R mod N: 2^(L-1) (< N) doubled RB*Sp - L + 1 times, 16 doublings per
+// mul_small, then one conditional subtraction.
+template <int Q, int G>
+__device__ __forceinline__ void r_mod_n(uint32_t (&one)[Q], const uint32_t (&n)[Q], int L) {
+  set_pow2<Q, G>(one, L - 1);
+  const double inv_n = 1.0 / scaled<Q, G>(n, L);
+  for (int steps = RB * Q * G - (L - 1); steps > 0; steps -= 16)
+    mul_small<Q, G>(one, 1u << (steps < 16 ? steps : 16), n, inv_n, L);
+  sub_if_ge<Q, G>(one, n);
 }
 
 // Exponentiation of x = Mont(base) by N-1 with the strong-test checks (the

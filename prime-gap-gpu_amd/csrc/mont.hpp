@@ -378,13 +378,14 @@ PGX_FN double scaled(const uint32_t (&v)[S], int L) {
   return d;
 }
 
-// x <- a x mod N for an ordinary integer 2 <= a < 2^8, which keeps the
-// Montgomery form (the step of a set exponent bit for a small base): x < 2N in
-// (limbs < 2^RB), x < 2N out.  q = floor(a x / N) is estimated in double
-// precision (relative error below S 2^-52 <= 2^-45) and biased down by 2^-40,
-// so it is the true quotient or one less (a x / N < 2^9) and a x - q N lies in
-// [0, 2N).  One pass forms a x_j - q n_j with a signed carry.  inv_n =
-// 1 / scaled(n, L).
+// x <- a x mod N for an ordinary integer 1 <= a <= 2^16, which keeps the
+// Montgomery form (the step of a set exponent bit for a small base, and 16
+// doublings at a time in r_mod_n): x < 2N in (limbs < 2^RB), x < 2N out.
+// q = floor(a x / N) is estimated in double precision (relative error below
+// 2 S 2^-53 <= 2^-45) and biased down by 2^-40, so it is the true quotient or
+// one less (a x / N < 2^17, so the bias moves it by less than 2^-22) and
+// a x - q N lies in [0, 2N).  One pass forms a x_j - q n_j with a signed carry
+// (|a x_j - q n_j| < 2^47).  inv_n = 1 / scaled(n, L).
 template <int S>
 PGX_FN void mul_small(uint32_t (&x)[S], uint32_t a, const uint32_t (&n)[S], double inv_n, int L) {
   constexpr int RB = Radix<S>::RB;
@@ -432,13 +433,18 @@ PGX_FN void set_pow2(uint32_t (&v)[S], int bit) {
   for (int j = 0; j < S; ++j) v[j] = (j == q) ? (1u << r) : 0u;
 }
 
-// one <- R mod N = 2^(RB*S) mod N, by doubling 2^(L-1) (which is < N).
+// one <- R mod N = 2^(RB*S) mod N: 2^(L-1) (which is < N) doubled
+// RB*S - L + 1 times, 16 doublings per mul_small (x < 2N throughout), then one
+// conditional subtraction.  (One doubling per step with a full compare cost
+// about 8 S instructions a doubling, 4,400 per 462-bit candidate.)
 template <int S>
 PGX_FN void r_mod_n(uint32_t (&one)[S], const uint32_t (&n)[S], int L) {
   constexpr int RB = Radix<S>::RB;
   set_pow2<S>(one, L - 1);
-  const int steps = RB * S - (L - 1);
-  for (int c = 0; c < steps; ++c) dbl_full<S>(one, n);
+  const double inv_n = 1.0 / scaled<S>(n, L);
+  for (int steps = RB * S - (L - 1); steps > 0; steps -= 16)
+    mul_small<S>(one, 1u << (steps < 16 ? steps : 16), n, inv_n, L);
+  sub_if_ge<S>(one, n);
 }
 
 // ---- conversion between 32-bit words and radix-2^RB limbs ----------------
