@@ -127,9 +127,9 @@ MIXED = [(211, 16), (337, 16), (503, 24), (701, 32)]  # (P, buffer words), d = 2
 # script with --copy-probe (profiles/k_prp_plain_s17_p337_d2310_2e20.md):
 # KiB per dispatch of FETCH_SIZE and WRITE_SIZE, each scaled by known bytes /
 # counter reading of the 256 MiB device copy in the same run (copy read
-# 262144 KiB, FETCH_SIZE read 131087.12; copy wrote 262144 KiB, WRITE_SIZE 262144).
+# 262144 KiB, FETCH_SIZE read 131087.19; copy wrote 262144 KiB, WRITE_SIZE 262144).
 TRAFFIC_KIB = {
-    "k_prp_plain<17,true>": {"fetch": 2192.43, "fetch_scale": 262144 / 131087.12,
+    "k_prp_plain<17,true>": {"fetch": 2202.42, "fetch_scale": 262144 / 131087.19,
                              "write": 4352.0, "write_scale": 1.0},
 }
 
