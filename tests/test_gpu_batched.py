@@ -82,6 +82,41 @@ def test_plan_graph_replays_match_eager():
     assert np.array_equal(eager[0].verdicts(), O.verdicts(L0, O.MODE_FERMAT))
 
 
+def test_plan_with_a_gpu_filling_bucket_beside_small_ones():
+    """A bucket of more than one wavefront per SIMD (81,920 candidates) runs on
+    the plan's capture stream, two small buckets on branches beside it; every
+    replay equals the eager result, and a sample of the large bucket the
+    oracle's."""
+    import torch
+    specs = [(337, 2310, 16, 20, 4096), (211, 210, 16, 4, 256), (503, 210, 24, 4, 256)]
+    data = [gen(P, d, w, n_m, per_m, seed_off=11) for P, d, w, n_m, per_m in specs]
+    assert len(data[0]) == 81920
+    batches = [pgx.DeviceBatch(L) for L in data]
+    eager = [pgx.DeviceBatch(L) for L in data]
+    with pgx.Runner(1, limbs=16) as run:
+        for e in eager:
+            run.test_persistent(e.bucket(), pgx.MODE_FERMAT, residues=True)
+        torch.cuda.synchronize()
+        say(f"plan over buckets of {[len(L) for L in data]}")
+        plan = pgx.Plan(run, [b.bucket() for b in batches], pgx.MODE_FERMAT, residues=True)
+        for rep in range(3):
+            for b in batches:
+                b.verdict.zero_()
+                b.res.zero_()
+            torch.cuda.synchronize()
+            plan.launch()
+            torch.cuda.synchronize()
+            for b, e in zip(batches, eager):
+                assert np.array_equal(b.verdicts(), e.verdicts()), rep
+                assert np.array_equal(b.residues(), e.residues()), rep
+            run.test_persistent(eager[1].bucket(), pgx.MODE_FERMAT, residues=True)
+            torch.cuda.synchronize()
+        plan.close()
+    ov, ores = O.verdicts(data[0][:512], O.MODE_FERMAT, residues=True)
+    assert np.array_equal(eager[0].verdicts()[:512], ov)
+    assert np.array_equal(eager[0].residues()[:512], ores)
+
+
 def test_fermat_then_mr_on_compacted_survivors():
     """Base-2 Fermat on every candidate, on-device compaction of the PRPs, then
     strong tests to bases 3 and 5 on the survivors only (P=907, d=30030)."""
