@@ -75,6 +75,26 @@ def test_tiny_brute_force(vectors):
     assert [int(x) for x in res[:, 0]] == [int(h, 16) for h in t["res"]]
 
 
+def test_tiny_n_in_wide_kernels_vs_oracle():
+    """Odd N < 4096 batched with one wide N, so that the kernel of the wide
+    one tests them all: R mod N then takes up to ~2,000 doublings (16 per
+    small-multiplier step) from 2^(L-1)."""
+    import random
+    rng = random.Random(5)
+    for bits in (987, 2019):
+        wide = rng.getrandbits(bits) | (1 << (bits - 1)) | 1
+        ns = list(range(3, 4096, 2)) + [wide]
+        words = (bits + 31) // 32
+        L = pgx.to_limbs(ns, words)
+        say(f"tiny N beside a {bits}-bit N: {len(ns)} candidates, {words} words")
+        ov, orr = O.verdicts(L, O.MODE_FERMAT, residues=True)
+        with pgx.Runner(len(ns), limbs=words, bases=[2, 3, 5]) as run:
+            v, res = run.run(L, pgx.MODE_FERMAT, residues=True)
+            assert np.array_equal(v, ov) and np.array_equal(res, orr), bits
+            assert np.array_equal(run.run(L, pgx.MODE_STRONG),
+                                  O.verdicts(L, O.MODE_STRONG, (2, 3, 5))), bits
+
+
 def test_all_odd_below_2_16_vs_oracle():
     ns = list(range(3, 1 << 16, 2))
     L = pgx.to_limbs(ns, 1)
