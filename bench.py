@@ -704,6 +704,81 @@ def bench_large(stream, peak: float, reps: int) -> dict:
             ev.record(st)
             stream.wait_event(ev)
 
+    # steady state: batch after batch of the same 2^18 candidates, two buffer
+    # sets; the Miller-Rabin stage of batch k (side streams) runs beside the
+    # Fermat launch of batch k+1, and batch k+2 reuses batch k's set once its
+    # Miller-Rabin stage is done.  A 1260-bit exponentiation takes ~8 ms at
+    # one wavefront per SIMD, a quarter of the Fermat launch, so the Fermat
+    # wavefronts that share SIMDs with survivors finish up to one such
+    # latency late whatever the stream layout (same queue, queues of their
+    # own, or the Miller-Rabin streams on a share of the compute units via
+    # pgx_stream_create_cus).
+    sets = []
+    for j in range(2):
+        sets.append({"b": a if j == 0 else pgx.DeviceBatch(L),
+                     "idx": torch.zeros(n, dtype=torch.int32, device="cuda"),
+                     "cnt": torch.zeros(64, dtype=torch.int32, device="cuda"),
+                     "v3": torch.zeros(n, dtype=torch.int32, device="cuda"),
+                     "v5": torch.zeros(n, dtype=torch.int32, device="cuda"), "done": []})
+
+    def pipeline_steady(batches, lanes, main, m3, m5, tf):
+        for k in range(batches):
+            S = sets[k % 2]
+            for ev in S["done"]:
+                main.wait_event(ev)
+            S["done"] = []
+            b = S["b"]
+            fermat.test_persistent(b.bucket(), pgx.MODE_FERMAT, residues=True, tuning=tf,
+                                   stream=main.cuda_stream)
+            fermat.compact(b.verdict.data_ptr(), n, pgx.FLAG_FERMAT, S["idx"].data_ptr(),
+                           S["cnt"].data_ptr(), main.cuda_stream)
+            ev = torch.cuda.Event()
+            ev.record(main)
+            for run, st, vv in ((r3, m3, S["v3"]), (r5, m5, S["v5"])):
+                st.wait_event(ev)
+                mb = pgx.Bucket(b.cand.data_ptr(), b.ld, b.limbs, n, b.max_bits, vv.data_ptr(), 0,
+                                b.ld)
+                run.test_indexed(mb, S["idx"].data_ptr(), S["cnt"].data_ptr(), pgx.MODE_STRONG,
+                                 tuning=pgx.Tuning(8, 4, 64, lanes), stream=st.cuda_stream)
+                done = torch.cuda.Event()
+                done.record(st)
+                S["done"].append(done)
+        for S in sets:
+            for ev in S["done"]:
+                main.wait_event(ev)
+            S["done"] = []
+
+    steady_batches = 8
+    steady = {}
+    steady_sets = []
+
+    def run_steady(key, lanes, main, m3, m5, tf):
+        pipeline_steady(2, lanes, main, m3, m5, tf)
+        steady[key] = time_ms(main, lambda: pipeline_steady(steady_batches, lanes, main, m3, m5,
+                                                            tf), 1) / steady_batches
+        torch.cuda.synchronize()
+        steady_sets.extend((S["v3"].cpu().numpy().view(np.uint32).copy(),
+                            S["v5"].cpu().numpy().view(np.uint32).copy()) for S in sets)
+
+    for lanes in (1, 16):
+        run_steady(f"lanes{lanes}", lanes, stream, s3, s5, t)
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    # every stage on a stream with a hardware queue of its own (a CU mask of
+    # every compute unit)
+    raw = [fermat.cu_stream(0, n_cu), r3.cu_stream(0, n_cu), r5.cu_stream(0, n_cu)]
+    main, m3, m5 = (torch.cuda.ExternalStream(x) for x in raw)
+    run_steady("lanes16_own_queues", 16, main, m3, m5, t)
+    for x, run in zip(raw, (fermat, r3, r5)):
+        run.destroy_stream(x)
+    # the Miller-Rabin streams on 12% of the compute units, Fermat on the rest
+    mr_cus = max(1, n_cu * 12 // 100)
+    raw = [fermat.cu_stream(mr_cus, n_cu - mr_cus), r3.cu_stream(0, mr_cus),
+           r5.cu_stream(0, mr_cus)]
+    main, m3, m5 = (torch.cuda.ExternalStream(x) for x in raw)
+    run_steady("lanes1_mr_on_12pct_cus", 1, main, m3, m5, t)
+    for x, run in zip(raw, (fermat, r3, r5)):
+        run.destroy_stream(x)
+
     pipeline()
     ms_fermat = time_ms(stream, fermat_only, reps)
     by_lanes = {}
@@ -734,7 +809,9 @@ def bench_large(stream, peak: float, reps: int) -> dict:
     ok = bool(np.array_equal(v[:1024], ov) and k == len(surv) and
               np.array_equal((f3[s_sample] == 1).astype(np.uint32) +
                              ((f3[s_sample] == 1) & (f5[s_sample] == 1)).astype(np.uint32), want)
-              and np.array_equal(f3c[surv], f3[surv]) and np.array_equal(f5c[surv], f5[surv]))
+              and np.array_equal(f3c[surv], f3[surv]) and np.array_equal(f5c[surv], f5[surv])
+              and all(np.array_equal(s3v[surv], f3[surv]) and np.array_equal(s5v[surv], f5[surv])
+                      for s3v, s5v in steady_sets))
     bits = median_bits(L)
     for r in (fermat, r3, r5):
         r.close()
@@ -752,6 +829,10 @@ def bench_large(stream, peak: float, reps: int) -> dict:
         "pipeline_chunked_ms": ms_chunked,
         "pipeline_chunked_tests_per_s": n / (ms_chunked * 1e-3),
         "pipeline_chunked_over_fermat": ms_chunked / ms_fermat,
+        "pipeline_steady_ms_per_batch": steady,
+        "pipeline_steady_best": min(steady, key=steady.get),
+        "pipeline_steady_over_fermat": min(steady.values()) / ms_fermat,
+        "pipeline_steady_batches": steady_batches,
         "mr_lanes_per_candidate": mr_lanes[0],
         "pipeline_ms_by_mr_lanes": {str(k): v for k, v in by_lanes.items()},
         "survivors": k, "prp_all_bases": int(len(prp)),

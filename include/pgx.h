@@ -259,6 +259,19 @@ int pgx_plan_create(pgx_plan_t** out, pgx_runner_t* h, const pgx_bucket_t* bucke
 int pgx_plan_launch(pgx_plan_t* p, void* stream);
 int pgx_plan_destroy(pgx_plan_t* p);
 
+/* A stream of the runner's device whose kernels run only on compute units
+ * [cu_first, cu_first + cu_count) (a CU mask; the stream gets a hardware
+ * queue of its own), for callers that split the GPU between concurrent
+ * stages: e.g. batch k+1's base-2 Fermat launch on most compute units and
+ * batch k's Miller-Rabin survivors on the rest, so that the latency-bound
+ * stage neither waits behind the big launch nor stretches it.  No
+ * counterpart in the reference (one stage per GPU thread there).  A process
+ * has few hardware queues: creation fails with PGX_E_HIP when none is left.
+ * The caller destroys the stream with pgx_stream_destroy once its work is
+ * done. */
+int pgx_stream_create_cus(pgx_runner_t* h, uint32_t cu_first, uint32_t cu_count, void** stream);
+int pgx_stream_destroy(pgx_runner_t* h, void* stream);
+
 /* ---- candidate construction: N = m * K + X with K = P# / d ----------------
  * Host-side builder for synthetic sieved candidates, the role of fill_batch
  * (gpu_testing.cu:136-195) and init_K (gap_common.cpp:60-65).

@@ -363,4 +363,27 @@ int pgx_plan_destroy(pgx_plan_t* p) {
   return PGX_OK;
 }
 
+int pgx_stream_create_cus(pgx_runner_t* h, uint32_t cu_first, uint32_t cu_count, void** out) {
+  if (!h || !out) return PGX_E_ARG;
+  *out = nullptr;
+  const uint32_t n_cu = (uint32_t)h->n_cu;
+  if (cu_count == 0 || cu_first >= n_cu || cu_count > n_cu - cu_first)
+    return fail(h, PGX_E_ARG, "compute units [%u, %u) outside the device's %u", cu_first,
+                cu_first + cu_count, n_cu);
+  PGX_HIP(h, hipSetDevice(h->device));
+  std::vector<uint32_t> mask((n_cu + 31) / 32, 0u);
+  for (uint32_t cu = cu_first; cu < cu_first + cu_count; ++cu) mask[cu / 32] |= 1u << (cu % 32);
+  hipStream_t s = nullptr;
+  PGX_HIP(h, hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()));
+  *out = s;
+  return PGX_OK;
+}
+
+int pgx_stream_destroy(pgx_runner_t* h, void* stream) {
+  if (!h || !stream) return PGX_E_ARG;
+  PGX_HIP(h, hipSetDevice(h->device));
+  PGX_HIP(h, hipStreamDestroy((hipStream_t)stream));
+  return PGX_OK;
+}
+
 }  // extern "C"

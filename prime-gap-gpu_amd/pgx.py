@@ -78,6 +78,8 @@ EXPORTED = (
     "pgx_plan_create",
     "pgx_plan_launch",
     "pgx_plan_destroy",
+    "pgx_stream_create_cus",
+    "pgx_stream_destroy",
     "pgx_serve_create",
     "pgx_serve_submit",
     "pgx_serve_poll",
@@ -203,6 +205,8 @@ def _load() -> ctypes.CDLL:
                                     ctypes.c_uint32]
     lib.pgx_plan_launch.argtypes = [vp, vp]
     lib.pgx_plan_destroy.argtypes = [vp]
+    lib.pgx_stream_create_cus.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(vp)]
+    lib.pgx_stream_destroy.argtypes = [vp, vp]
     lib.pgx_serve_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(ServeConfig)]
     lib.pgx_serve_submit.argtypes = [vp, ctypes.POINTER(ServeRequest)]
     lib.pgx_serve_poll.argtypes = [vp, ctypes.POINTER(ServeResult), ctypes.c_uint32,
@@ -491,6 +495,17 @@ class Runner:
         self._check(lib.pgx_compact(self._h, ctypes.c_void_p(verdict_ptr), n, mask,
                                     ctypes.c_void_p(index_ptr), ctypes.c_void_p(count_ptr),
                                     ctypes.c_void_p(stream or None)))
+
+    def cu_stream(self, cu_first: int, cu_count: int) -> int:
+        """pgx_stream_create_cus: a stream (raw hipStream_t as an int) whose
+        kernels run on compute units [cu_first, cu_first + cu_count) only;
+        release it with destroy_stream."""
+        out = ctypes.c_void_p()
+        self._check(lib.pgx_stream_create_cus(self._h, cu_first, cu_count, ctypes.byref(out)))
+        return int(out.value)
+
+    def destroy_stream(self, stream: int) -> None:
+        self._check(lib.pgx_stream_destroy(self._h, ctypes.c_void_p(stream)))
 
     def autotune(self, bucket: Bucket, mode: int) -> tuple[Tuning, float]:
         t = Tuning()

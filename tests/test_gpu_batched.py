@@ -274,6 +274,33 @@ def test_bad_tuning_and_bucket_are_rejected():
         assert e.value.code == pgx.PGX_E_INPUT
 
 
+def test_streams_on_a_share_of_the_compute_units():
+    """pgx_stream_create_cus: launches on a stream confined to a few compute
+    units give the oracle's results; ranges outside the device are rejected.
+    (Each such stream takes a hardware queue of its own, of which a process
+    has few: one stream here.)"""
+    import torch
+    L = gen(337, 2310, 16, 4, 1024, seed_off=3)
+    ov, ores = O.verdicts(L[:1024], O.MODE_FERMAT, residues=True)
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    with pgx.Runner(1, limbs=16) as run:
+        for first, count in ((n_cu - 8, 8),):
+            say(f"stream on compute units [{first}, {first + count}) of {n_cu}")
+            st = run.cu_stream(first, count)
+            b = pgx.DeviceBatch(L)
+            run.test_persistent(b.bucket(), pgx.MODE_FERMAT, residues=True, stream=st)
+            run.test_device(b.cand.data_ptr(), b.ld, b.limbs, 1024, b.max_bits, pgx.MODE_FERMAT,
+                            b.verdict.data_ptr(), b.res.data_ptr(), b.ld, st)
+            torch.cuda.synchronize()
+            run.destroy_stream(st)
+            assert np.array_equal(b.verdicts()[:1024], ov)
+            assert np.array_equal(b.residues()[:1024], ores)
+        for first, count in ((0, 0), (n_cu, 1), (n_cu - 1, 2)):
+            with pytest.raises(pgx.PgxError) as e:
+                run.cu_stream(first, count)
+            assert e.value.code == pgx.PGX_E_ARG
+
+
 def test_base_range_is_checked():
     with pytest.raises(pgx.PgxError) as e:
         pgx.Runner(16, limbs=16, bases=[2, 256])
