@@ -22,8 +22,10 @@ assemble (a short branch pushed past its 16-bit range) stays as compiled.
 `verify` disassembles both objects and checks, function by function, that
 after dropping `s_nop` and the _e32/_e64 suffix they hold the same
 instructions in the same order, that every branch lands on the same
-instruction, and that the wait states between any two instructions did not
-shrink; and that no 8-byte instruction of the aligned object sits at 4 mod 8.
+instruction, and that in front of every instruction there are at least the
+wait states the compiler's `s_nop`s put there (counted in its assembly when
+given, since alignment fill looks the same in an object); and that no
+8-byte instruction of the aligned object sits at 4 mod 8.
 It exits non-zero otherwise, which fails the build."""
 import re
 import subprocess
@@ -202,7 +204,7 @@ def build(dev_s, out_o):
         skip |= bad
     else:
         raise SystemExit('align_isa: gave up after 8 passes')
-    n = verify(orig_o, out_o, skip)
+    n = verify(orig_o, out_o, skip, dev_s)
     return stats, n, skip
 
 
@@ -292,21 +294,54 @@ def _canon(fn):
     return canon, waits
 
 
-def verify(orig, new, skip=()):
+def _source_waits(src):
+    """Per function of the compiler's assembly, the wait states its explicit
+    `s_nop`s put in front of each other instruction.  In the object the fill
+    of an alignment directive (`s_nop 0`s) looks the same, and its amount
+    depends on the code before it, so the required wait states come from the
+    source."""
+    lines = open(src).read().split('\n')
+    funcs = {m.group(1) for m in (re.match(r'\s*\.type\s+([^,\s]+)\s*,\s*@function', l)
+                                  for l in lines) if m}
+    out, cur, pending = {}, None, 0
+    for line in lines:
+        t = line.split(';')[0].strip()
+        if t.endswith(':') and t[:-1] in funcs:
+            cur, pending = out.setdefault(t[:-1], []), 0
+            continue
+        if t.startswith('.Lfunc_end'):
+            cur = None
+        if cur is None or not t or t.startswith(('.', '//')) or t.endswith(':'):
+            continue
+        if t.split()[0] == 's_nop':
+            pending += int(_operands(t)[0], 0) + 1
+            continue
+        cur.append(pending)
+        pending = 0
+    return out
+
+
+def verify(orig, new, skip=(), src=None):
     a, b = _disasm(orig), _disasm(new)
     if set(a) != set(b):
         raise SystemExit(f'align_isa: functions differ: {sorted(set(a) ^ set(b))[:5]}')
+    need = _source_waits(src) if src else {}
     misaligned = 0
     for name in a:
         ca, wa = _canon(a[name])
         cb, wb = _canon(b[name])
         if len(ca) != len(cb):
             raise SystemExit(f'align_isa: {name}: {len(ca)} vs {len(cb)} instructions')
+        req = need.get(name)
+        if req is not None and len(req) != len(ca):
+            raise SystemExit(f'align_isa: {name}: {len(req)} instructions in the source, '
+                             f'{len(ca)} in the object')
         for i, (x, y) in enumerate(zip(ca, cb)):
             if x != y:
                 raise SystemExit(f'align_isa: {name}: instruction {i}: {x} vs {y}')
-            if wb[i] < wa[i]:
-                raise SystemExit(f'align_isa: {name}: instruction {i}: wait states {wa[i]} -> {wb[i]}')
+            w = req[i] if req is not None else wa[i]
+            if wb[i] < w:
+                raise SystemExit(f'align_isa: {name}: instruction {i}: wait states {w} -> {wb[i]}')
         if name in skip:
             continue
         base = b[name]['base']
