@@ -107,6 +107,40 @@ def test_all_odd_below_2_16_vs_oracle():
             assert np.array_equal(run.run(L, mode), O.verdicts(L, mode, (2, 3, 5)))
 
 
+def test_headline_2e20_all_vs_gmp():
+    """The headline workload whole: 2^20 sieved candidates (P=337, d=2310, 64
+    multipliers, X <= 450000, 16-limb buffers, the benchmark's seed) in 16
+    plain launches of 65,536 on HBM-resident buffers; every verdict and
+    residue against GMP (the oracle on all host cores, chunk by chunk)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    import torch
+    m = pgx.gen_multipliers(0x5EED0002, 10**7, 10**7 + 10**4, 2310, 64)
+    L, _, _ = pgx.gen_sieved(337, 2310, m, 450000, 16384, 16)
+    n = len(L)
+    assert n == 1 << 20
+    say(f"headline: {n} candidates, 16 launches of 65536")
+    b = pgx.DeviceBatch(L)
+    with pgx.Runner(1, limbs=16) as run:
+        for c0 in range(0, n, 65536):
+            run.test_device(b.cand.data_ptr() + 4 * c0, b.ld, b.limbs, 65536, b.max_bits,
+                            pgx.MODE_FERMAT, b.verdict.data_ptr() + 4 * c0,
+                            b.res.data_ptr() + 4 * c0, b.ld)
+        torch.cuda.synchronize()
+    v, res = b.verdicts(), b.residues()
+    chunks = [(c0, min(c0 + 32768, n)) for c0 in range(0, n, 32768)]
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    say(f"oracle on {threads} threads")
+    with ThreadPoolExecutor(threads) as ex:
+        out = list(ex.map(lambda c: O.verdicts(L[c[0]:c[1]], O.MODE_FERMAT, residues=True),
+                          chunks))
+    ov = np.concatenate([o[0] for o in out])
+    orr = np.concatenate([o[1] for o in out])
+    assert np.array_equal(v, ov)
+    assert np.array_equal(res, orr)
+
+
 @pytest.mark.parametrize("P,d,limbs,per_m", [(337, 2310, 16, 1024), (211, 210, 16, 256),
                                              (503, 210, 24, 256), (701, 210, 32, 256),
                                              (907, 30030, 64, 128)])
