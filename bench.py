@@ -21,6 +21,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -389,7 +390,9 @@ def bench_dropin(n_x: int = 8, passes: int = 3) -> dict:
     pairs = np.frombuffer(raw, dtype="<u4", count=2 * nx, offset=4).reshape(nx, 2)
     (w,) = struct.unpack_from("<I", raw, 4 + 8 * nx)
     found = np.frombuffer(raw, dtype="<u4", count=w, offset=8 + 8 * nx)
-    secs = [float(v) for v in r.stdout.split("seconds per pass:")[1].split()]
+    line = r.stdout.split("seconds per pass:")[1]
+    secs = [float(v) for v in line.split(";")[0].split()]
+    stages = dict(re.findall(r"(fill|queued|run|results) ([0-9.]+)", line))
     tests = int(pairs[:, 0].sum())
     equal = bool([tuple(int(v) for v in p) for p in pairs] == [tuple(x) for x in want]
                  and np.array_equal(found, want_found[:w]) and not want_found[w:].any())
@@ -402,6 +405,7 @@ def bench_dropin(n_x: int = 8, passes: int = 3) -> dict:
         "seconds_per_pass": secs,
         "tests_per_s": tests / float(np.mean(timed)),
         "equal_to_device_search": equal,
+        "stage_seconds_all_passes": {k: float(v) for k, v in stages.items()},
         "harness_line": r.stdout.strip().splitlines()[-1],
     }
 

@@ -51,8 +51,18 @@ static const uint32_t kBits = 1024;
 #endif
 static const uint32_t kLimbs = kBits / 32;
 
+// Build-time settings, like GPU_BITS: -DPGX_GPU_BATCHES=k, -DPGX_GPU_BATCH_SIZE=n
+// (n one of the sizes prime_gap_test accepts).
+#ifdef PGX_GPU_BATCHES
+const size_t GPU_BATCHES = PGX_GPU_BATCHES;
+#else
 const size_t GPU_BATCHES = 3;
+#endif
+#ifdef PGX_GPU_BATCH_SIZE
+const size_t GPU_BATCH_SIZE = PGX_GPU_BATCH_SIZE;
+#else
 const size_t GPU_BATCH_SIZE = 32 * 1024;
+#endif
 static const uint32_t kRounds = 1;  // base-2 strong test, as the reference's ROUNDS
 
 namespace {

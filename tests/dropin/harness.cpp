@@ -168,11 +168,14 @@ bool bit(const std::vector<uint32_t>& bm, uint32_t m_i) {
   return (bm[t >> 5] >> (t & 31)) & 1u;
 }
 
-// remove_vector (gap_search_gpu.cpp:340-351): drop every a whose bit is set in B.
+// remove_vector (gap_search_gpu.cpp:340-351): drop every a whose bit is set in
+// B, branch-free as there (every a is written, the index advances by 0 or 1).
 void remove_found(std::vector<uint32_t>& A, const std::vector<uint32_t>& B) {
   size_t i = 0;
-  for (uint32_t a : A)
-    if (!bit(B, a)) A[i++] = a;
+  for (uint32_t a : A) {
+    A[i] = a;
+    i += !bit(B, a);
+  }
   A.resize(i);
 }
 
@@ -335,6 +338,10 @@ int main(int argc, char** argv) {
          " seconds per pass:", passes, in.xs.size(), (unsigned long)last.total_tests,
          (size_t)test_data.gpu_stats.batches_run, GPU_BATCH_SIZE, GPU_BATCHES);
   for (double s : secs) printf(" %.6f", s);
-  printf("\n");
+  // per-stage seconds summed over every batch of every pass (GpuStatsCounters)
+  const GpuStatsCounters& g = test_data.gpu_stats;
+  printf("; stage seconds over all passes: fill %.6f queued %.6f run %.6f results %.6f"
+         " (%lu batches, %lu partial)\n", g.d_fill, g.d_queued_full, g.d_run, g.d_results,
+         (unsigned long)g.batches_run, (unsigned long)g.batches_partial);
   return 0;
 }
