@@ -138,7 +138,12 @@ TRAFFIC_KIB = {
 def bench_mixed(stream, peak: float, reps: int) -> dict:
     """2^21 candidates, P in {211, 337, 503, 701}, d=210: one hipGraph launch
     over four autotuned persistent buckets vs the plain kernel (one launch per
-    65536) on the same HBM-resident data."""
+    65536) on the same HBM-resident data.  Beside them, the plain kernel at one
+    width for the whole mix, as the reference runs a job (one compile-time
+    BITS, gpu_testing.cu:43-62; a CGBN instance of that size for every
+    candidate): the widest bucket's kernel over the same per-P slices, and over
+    the candidates interleaved P by P in 32-word buffers (an unbucketed
+    stream: every wavefront then holds a 987-bit candidate)."""
     import torch
 
     import oracle
@@ -175,8 +180,40 @@ def bench_mixed(stream, peak: float, reps: int) -> dict:
     plain()
     ms_plain = time_ms(stream, plain, max(1, reps // 2))
     total = sum(len(L) for L in data)
-    macs = sum(work_per_test(median_bits(L)) * len(L) for L in data)
+    widest = max(b.max_bits for b in batches)
+
+    def one_width():
+        for b in batches:
+            for c0 in range(0, b.n, BATCH):
+                k = min(BATCH, b.n - c0)
+                runner.test_device(b.cand.data_ptr() + 4 * c0, b.ld, b.limbs, k, widest,
+                                   pgx.MODE_FERMAT, b.verdict.data_ptr() + 4 * c0,
+                                   b.res.data_ptr() + 4 * c0, b.ld, stream.cuda_stream)
+    one_width()
+    ms_one_width = time_ms(stream, one_width, max(1, reps // 2))
+    ok_one_width = all(
+        np.array_equal(b.verdicts()[:1024], oracle.verdicts(L[:1024], oracle.MODE_FERMAT))
+        for b, L in zip(batches, data))
+    # interleaved P by P (equal counts per P), zero-padded to 32 words
+    per = min(len(L) for L in data)
+    mixed = np.zeros((per, len(data), 32), dtype=np.uint32)
+    for q, L in enumerate(data):
+        mixed[:, q, :L.shape[1]] = L[:per]
+    mixed = mixed.reshape(per * len(data), 32)
     plan.close()
+    ub = pgx.DeviceBatch(mixed, max_bits=widest)
+
+    def unbucketed():
+        for c0 in range(0, ub.n, BATCH):
+            k = min(BATCH, ub.n - c0)
+            runner.test_device(ub.cand.data_ptr() + 4 * c0, ub.ld, ub.limbs, k, widest,
+                               pgx.MODE_FERMAT, ub.verdict.data_ptr() + 4 * c0,
+                               ub.res.data_ptr() + 4 * c0, ub.ld, stream.cuda_stream)
+    unbucketed()
+    ms_unbucketed = time_ms(stream, unbucketed, max(1, reps // 2))
+    ok_one_width &= bool(np.array_equal(
+        ub.verdicts()[:1024], oracle.verdicts(mixed[:1024], oracle.MODE_FERMAT)))
+    macs = sum(work_per_test(median_bits(L)) * len(L) for L in data)
     runner.close()
     cpu = [cpu_rate(L, 1.5) for L in data]
     cpu_s = sum(len(L) / c["tests_per_s"] for L, c in zip(data, cpu))
@@ -187,6 +224,18 @@ def bench_mixed(stream, peak: float, reps: int) -> dict:
         "graph_ms": ms_graph, "graph_tests_per_s": total / (ms_graph * 1e-3),
         "plain_ms": ms_plain, "plain_tests_per_s": total / (ms_plain * 1e-3),
         "speedup_vs_plain": ms_plain / ms_graph,
+        "one_width_plain": {
+            "kernel_limbs": kernel_limbs(widest),
+            "per_p_slices_ms": ms_one_width,
+            "per_p_slices_tests_per_s": total / (ms_one_width * 1e-3),
+            "unbucketed_candidates": per * len(data),
+            "unbucketed_ms": ms_unbucketed,
+            "unbucketed_tests_per_s": per * len(data) / (ms_unbucketed * 1e-3),
+            "graph_speedup_vs_per_p_slices": ms_one_width / ms_graph,
+            "graph_speedup_vs_unbucketed": (ms_unbucketed / (per * len(data))) /
+                                           (ms_graph / total),
+            "parity_sample_ok": bool(ok_one_width),
+        },
         "valu_frac": macs / (ms_graph * 1e-3) / peak,
         "buckets": per_bucket,
         "parity_sample_ok": ok,
@@ -1107,6 +1156,9 @@ def summary(plain_512: float, frac_512, extra: dict) -> dict:
         "tests_per_s_512_dropin_three_gpu_threads": get("dropin", "tests_per_s"),
         "tests_per_s_2048_lanes16": get("class_2048", "lanes16_tests_per_s"),
         "tests_per_s_2048_lanes32": get("class_2048", "lanes32_tests_per_s"),
+        "mixed_graph_speedup_vs_one_width_unbucketed":
+            (extra.get("mixed_graph", {}).get("one_width_plain") or {})
+            .get("graph_speedup_vs_unbucketed"),
         "serving_capacity_per_s_p99_within_20ms": (extra.get("serving", {}).get("capacity") or {})
         .get("max_offered_per_s_within_limit"),
     }
