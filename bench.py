@@ -385,7 +385,7 @@ def bench_host_boundary(cands: np.ndarray, reps: int = 3) -> dict:
     return out
 
 
-def bench_dropin(n_x: int = 8, passes: int = 3) -> dict:
+def bench_dropin(n_x: int = 8, passes: int = 3, keep_dir: str | None = None) -> dict:
     """The drop-in gpu_testing.cpp run by tests/dropin/harness under the
     reference's own dispatch protocol: GPU_BATCHES = 3 host threads, each with
     a runner and a GPUBatch of up to 32768 multipliers (candidates built on the
@@ -394,7 +394,8 @@ def bench_dropin(n_x: int = 8, passes: int = 3) -> dict:
     from the device sieve by the primes below 1e8, first n_x coprime X.  PCIe
     and host work included.  Pass 1 warms up (runner creation); the rate is
     over the later passes.  Cross-checked X by X against pgx_search_step over
-    the same interval."""
+    the same interval.  With keep_dir the harness's input and output files
+    stay there (in.bin, out.bin), e.g. for a profiled rerun of the harness."""
     import struct
     import subprocess
     import tempfile
@@ -421,7 +422,10 @@ def bench_dropin(n_x: int = 8, passes: int = 3) -> dict:
             want = [s.step(X) for X in xs]
             want_found = s.bitmap(pgx.SEARCH_FOUND)
     with tempfile.TemporaryDirectory() as tmp:
-        inp, out = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+        if keep_dir:
+            os.makedirs(keep_dir, exist_ok=True)
+        where = keep_dir or tmp
+        inp, out = os.path.join(where, "in.bin"), os.path.join(where, "out.bin")
         with open(inp, "wb") as f:
             f.write(b"PGXH")
             f.write(struct.pack("<IIQIIII", P, d, m_start, m_inc, len(xs), len(comps[0]),
@@ -1184,6 +1188,8 @@ def main() -> int:
     ap.add_argument("--plain-classes", default="1024,1260,2048",
                     help="comma-separated width classes (1024,1260,2048) to time through the "
                          "plain kernel alone, one entry per class in configs")
+    ap.add_argument("--dropin-dir", default=None,
+                    help="keep the drop-in harness's input and output files in this folder")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -1309,7 +1315,7 @@ def main() -> int:
             f"{hb['cpu_build_candidates_per_s_one_thread'] / 1e6:.2f} M/s "
             f"(equal: {hb['build_equals_gmp']})")
     if "dropin" in todo:
-        extra["dropin"] = bench_dropin()
+        extra["dropin"] = bench_dropin(keep_dir=args.dropin_dir)
         dr = extra["dropin"]
         log(f"drop-in: {dr.get('tests_per_s', 0) / 1e6:.2f} M tests/s through gpu_testing.cpp "
             f"on 3 GPU threads (equal to the device search: {dr.get('equal_to_device_search')})"
