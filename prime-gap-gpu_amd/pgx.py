@@ -336,6 +336,54 @@ def gen_requests(P: int, d: int, n_req: int, seed: int, limbs: int, lo: int = 20
     return out
 
 
+SERVE_LOAD = os.path.join(_HERE, "serve_load")
+
+
+def write_serve_templates(path: str, templates) -> None:
+    """Request templates for serve_load: templates = [(max_bits, next_limbs,
+    prev_limbs)], each side candidate-major limbs of one width, ascending X."""
+    import struct
+    with open(path, "wb") as f:
+        f.write(b"PGXS")
+        f.write(struct.pack("<I", len(templates)))
+        for max_bits, nxt, prv in templates:
+            nxt = np.ascontiguousarray(nxt, dtype="<u4")
+            prv = np.ascontiguousarray(prv, dtype="<u4")
+            limbs = nxt.shape[1] if nxt.ndim == 2 else prv.shape[1]
+            f.write(struct.pack("<IIII", limbs, int(max_bits), len(nxt), len(prv)))
+            f.write(nxt.tobytes())
+            f.write(prv.tobytes())
+
+
+def run_serve_load(templates_path: str, rate: float, duration_s: float, seed: int,
+                   timeout: float = 300, **cfg) -> dict:
+    """Runs serve_load (the C++ Poisson load generator over pgx_serve_*) and
+    reads its results: per request (in id order) the template index, the first
+    PRP index of each side, tests and latency in ns, plus its summary line.
+    cfg: pgx_serve_config_t fields (hbm_gb, chunk, group_lanes, ...)."""
+    import subprocess
+    import tempfile
+    if not os.path.exists(SERVE_LOAD):
+        raise ImportError(f"{SERVE_LOAD} is not built (run make in {_HERE})")
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "out.bin")
+        args = [SERVE_LOAD, templates_path, out, repr(float(rate)), repr(float(duration_s)),
+                str(int(seed))] + [f"{k}={int(v)}" for k, v in cfg.items()]
+        r = subprocess.run(args, capture_output=True, text=True, errors="replace",
+                           timeout=timeout)
+        if r.returncode != 0:
+            raise PgxError(PGX_E_HIP, f"serve_load exited {r.returncode}: "
+                                      f"{(r.stdout + r.stderr)[-2000:]}")
+        raw = open(out, "rb").read()
+    n = int(np.frombuffer(raw, dtype="<u4", count=1)[0])
+    rec = np.frombuffer(raw, dtype=np.dtype([("template", "<u4"), ("next_first", "<i4"),
+                                             ("prev_first", "<i4"), ("tests", "<u4"),
+                                             ("latency_ns", "<u8")]), count=n, offset=4)
+    return {"template": rec["template"].copy(), "next_first": rec["next_first"].copy(),
+            "prev_first": rec["prev_first"].copy(), "tests": rec["tests"].copy(),
+            "latency_ns": rec["latency_ns"].copy(), "line": r.stdout.strip().splitlines()[-1]}
+
+
 VALU_OPS = {0: "v_mad_u64_u32", 1: "v_mad_u64_u32_dependent", 2: "v_add_u32", 3: "v_mul_lo_u32",
             4: "v_fma_f64"}
 

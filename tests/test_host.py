@@ -298,3 +298,50 @@ def test_counter_slots_two_threads_share_one_runner(tmp_path):
     # afterwards the slots keep coming round in order, each once per 63
     rest = vals[63:]
     assert all(rest[i] % 63 + 1 == rest[i + 1] for i in range(len(rest) - 1))
+
+
+def _serve_load(*args):
+    if not os.path.exists(pgx.SERVE_LOAD):
+        pytest.skip("prime-gap-gpu_amd/serve_load is not built (build())")
+    return subprocess.run([pgx.SERVE_LOAD, *map(str, args)], capture_output=True, text=True,
+                          timeout=60)
+
+
+def test_serve_load_rejects_bad_arguments_before_any_gpu_call(tmp_path):
+    """The C++ serving load generator exits 2 on bad arguments or template
+    files, before it creates a server (so this runs without a GPU)."""
+    assert _serve_load().returncode == 2
+    assert _serve_load(tmp_path / "missing.bin", tmp_path / "o", 1000, 1, 1).returncode == 2
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"XXXX\x00\x00\x00\x00")
+    assert _serve_load(bad, tmp_path / "o", 1000, 1, 1).returncode == 2
+    good = tmp_path / "t.bin"
+    cands = pgx.to_limbs([2**61 - 1, 2**89 - 1], 4)
+    pgx.write_serve_templates(str(good), [(89, cands, cands[:1])])
+    r = _serve_load(good, tmp_path / "o", 0, 1, 1)  # no requests
+    assert r.returncode == 2 and "RATE" in r.stderr
+    r = _serve_load(good, tmp_path / "o", 1000, 1, 1, "colour=blue")
+    assert r.returncode == 2 and "unknown key" in r.stderr
+    # a truncated template (header promises more candidates than the file holds)
+    raw = good.read_bytes()
+    short = tmp_path / "short.bin"
+    short.write_bytes(raw[:-4])
+    assert _serve_load(short, tmp_path / "o", 1000, 1, 1).returncode == 2
+
+
+def test_serve_templates_layout(tmp_path):
+    """write_serve_templates: "PGXS", count, then per template limbs, max_bits,
+    n_next, n_prev and the candidate-major words of each side."""
+    import struct
+    nxt = pgx.to_limbs([2**61 - 1, 2**89 - 1, 2**107 - 1], 4)
+    prv = pgx.to_limbs([2**31 - 1], 4)
+    path = tmp_path / "t.bin"
+    pgx.write_serve_templates(str(path), [(107, nxt, prv), (107, prv, nxt[:0])])
+    raw = path.read_bytes()
+    assert raw[:4] == b"PGXS" and struct.unpack_from("<I", raw, 4)[0] == 2
+    assert struct.unpack_from("<IIII", raw, 8) == (4, 107, 3, 1)
+    words = np.frombuffer(raw, dtype="<u4", count=16, offset=24)
+    assert np.array_equal(words[:12].reshape(3, 4), nxt)
+    assert np.array_equal(words[12:].reshape(1, 4), prv)
+    assert struct.unpack_from("<IIII", raw, 24 + 64) == (4, 107, 1, 0)
+    assert len(raw) == 24 + 64 + 16 + 16
