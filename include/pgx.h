@@ -55,6 +55,18 @@ int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t lim
 /* Free everything the runner owns.  Replaces miller_rabin.h:351-362. */
 int pgx_runner_destroy(pgx_runner_t* h);
 
+/* Confines the runner's kernels to compute units [part*n/parts,
+ * (part+1)*n/parts) of its device's n (a CU mask on its streams; parts = 1
+ * lifts it).  For callers that keep several runners' batches in flight at
+ * once, as the reference's GPU_BATCHES threads do (gpu_testing.cu:203-354, one
+ * runner and stream each): unconfined, their kernels share SIMDs and the
+ * older wavefronts take the issue slots, so concurrent batches end one after
+ * another; confined, each runs on its own CUs.  No counterpart in the
+ * reference.  Only between batches (PGX_E_STATE while one is in flight); work
+ * queued on the runner's own stream finishes first.  PGX_E_HIP when the
+ * process has no hardware queue left for a masked stream. */
+int pgx_runner_confine(pgx_runner_t* h, uint32_t part, uint32_t parts);
+
 /* Copy n candidates (candidate-major limbs, see above) into pinned staging,
  * send them to the device as they are (a device kernel makes them
  * limb-interleaved) and launch asynchronously.  The caller's array may be

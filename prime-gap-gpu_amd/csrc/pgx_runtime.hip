@@ -182,13 +182,20 @@ void release(pgx_runner* h) {
   if (h->d_counters) (void)hipFree(h->d_counters);
 }
 
+// A stream for the runner's batches: non-blocking, on the CUs of its mask
+// (every CU when the mask is empty).
+hipError_t batch_stream(const pgx_runner* h, hipStream_t* s) {
+  if (h->cu_mask.empty()) return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+  return hipExtStreamCreateWithCUMask(s, (uint32_t)h->cu_mask.size(), h->cu_mask.data());
+}
+
 // A free batch slot, in submit order: slot 0, or the second slot (made here
 // the first time slot 0 is busy); NULL when both hold a batch.
 pgx_batch* acquire_batch(pgx_runner* h) {
   if (!h->b0.pending) return &h->b0;
   if (!h->b1) {
     auto b = std::make_unique<pgx_batch>();
-    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    hipError_t e = batch_stream(h, &b->stream);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_count, 4);
     if (e == hipSuccess) e = alloc_batch(h, *b);
     if (e != hipSuccess) {
@@ -371,6 +378,43 @@ int pgx_runner_create(pgx_runner_t** out, int device, size_t max_n, uint32_t lim
   }
   h->err = "no error";
   *out = h;
+  return PGX_OK;
+}
+
+int pgx_runner_confine(pgx_runner_t* h, uint32_t part, uint32_t parts) {
+  if (!h) return PGX_E_ARG;
+  const uint32_t n_cu = (uint32_t)h->n_cu;
+  if (parts == 0 || part >= parts || parts > n_cu)
+    return fail(h, PGX_E_ARG, "part %u of %u: need part < parts <= %u compute units", part, parts,
+                n_cu);
+  if (!h->inflight.empty()) return fail(h, PGX_E_STATE, "a batch is in flight");
+  PGX_HIP(h, hipSetDevice(h->device));
+  std::vector<uint32_t> mask;
+  if (parts > 1) {
+    mask.assign((n_cu + 31) / 32, 0u);
+    for (uint32_t cu = part * n_cu / parts; cu < (part + 1) * n_cu / parts; ++cu)
+      mask[cu / 32] |= 1u << (cu % 32);
+  }
+  // new streams first (nothing changes if that fails), then the old ones go
+  std::swap(h->cu_mask, mask);
+  hipStream_t s0 = nullptr, s1 = nullptr;
+  hipError_t e = batch_stream(h, &s0);
+  if (e == hipSuccess && h->b1) e = batch_stream(h, &s1);
+  if (e != hipSuccess) {
+    if (s0) (void)hipStreamDestroy(s0);
+    std::swap(h->cu_mask, mask);
+    return fail(h, PGX_E_HIP, "stream on compute units: %s", hipGetErrorString(e));
+  }
+  // work the caller queued on the runner's stream (pgx_test_device with no
+  // stream of its own) finishes on the old stream
+  PGX_HIP(h, hipStreamSynchronize(h->stream));
+  (void)hipStreamDestroy(h->stream);
+  h->stream = h->b0.stream = s0;
+  if (h->b1) {
+    PGX_HIP(h, hipStreamSynchronize(h->b1->stream));
+    (void)hipStreamDestroy(h->b1->stream);
+    h->b1->stream = s1;
+  }
   return PGX_OK;
 }
 

@@ -57,6 +57,8 @@ struct pgx_runner {
   uint32_t limbs = 0;
   pgx::Bases bases{};
   hipStream_t stream = nullptr;    // the runner's stream (also batch slot 0's)
+  // CU mask of the runner's streams (pgx_runner_confine); empty: every CU
+  std::vector<uint32_t> cu_mask;
   uint32_t* d_counters = nullptr;  // work counters of persistent launches
   pgx::CounterSlots slots{pgx::kCounters};  // the slot each persistent launch takes
   pgx_batch b0;                    // slot 0: the m-interval search borrows its buffers

@@ -63,6 +63,15 @@ const size_t GPU_BATCH_SIZE = PGX_GPU_BATCH_SIZE;
 #else
 const size_t GPU_BATCH_SIZE = 32 * 1024;
 #endif
+// -DPGX_GPU_CONFINE=1 puts each GPU thread's runner on its own share of the
+// CUs (pgx_runner_confine).  Off by default: under this dispatch it was faster
+// only with 2 threads and 4 hardware queues per process (28.0 against 37.7 ms
+// per pass), slower with 3, 4 or 6 threads or with 16 queues.
+#ifdef PGX_GPU_CONFINE
+static const bool kConfine = PGX_GPU_CONFINE;
+#else
+static const bool kConfine = false;
+#endif
 static const uint32_t kRounds = 1;  // base-2 strong test, as the reference's ROUNDS
 
 namespace {
@@ -133,6 +142,10 @@ void run_gpu_thread(int runner_num, int verbose, TestData& test_data, GPUBatch& 
     std::vector<uint32_t> verdict(GPU_BATCH_SIZE);
     if (pgx_runner_create(&runner, 0, GPU_BATCH_SIZE, kLimbs, nullptr, kRounds) != PGX_OK)
       throw std::runtime_error(std::string("pgx_runner_create: ") + pgx_last_error(nullptr));
+    // each GPU thread's batches on their own share of the CUs
+    if (kConfine && GPU_BATCHES > 1 &&
+        pgx_runner_confine(runner, (uint32_t)runner_num % GPU_BATCHES, GPU_BATCHES) != PGX_OK)
+      throw std::runtime_error(std::string("pgx_runner_confine: ") + pgx_last_error(runner));
 
     size_t processed = 0;
     while (is_running && stop_queue <= 1) {

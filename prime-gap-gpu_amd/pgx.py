@@ -80,6 +80,7 @@ EXPORTED = (
     "pgx_plan_destroy",
     "pgx_stream_create_cus",
     "pgx_stream_destroy",
+    "pgx_runner_confine",
     "pgx_serve_create",
     "pgx_serve_submit",
     "pgx_serve_poll",
@@ -207,6 +208,7 @@ def _load() -> ctypes.CDLL:
     lib.pgx_plan_destroy.argtypes = [vp]
     lib.pgx_stream_create_cus.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(vp)]
     lib.pgx_stream_destroy.argtypes = [vp, vp]
+    lib.pgx_runner_confine.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
     lib.pgx_serve_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(ServeConfig)]
     lib.pgx_serve_submit.argtypes = [vp, ctypes.POINTER(ServeRequest)]
     lib.pgx_serve_poll.argtypes = [vp, ctypes.POINTER(ServeResult), ctypes.c_uint32,
@@ -554,6 +556,11 @@ class Runner:
 
     def destroy_stream(self, stream: int) -> None:
         self._check(lib.pgx_stream_destroy(self._h, ctypes.c_void_p(stream)))
+
+    def confine(self, part: int, parts: int) -> None:
+        """pgx_runner_confine: the runner's kernels on the part-th of `parts`
+        equal compute-unit ranges (parts = 1: every CU)."""
+        self._check(lib.pgx_runner_confine(self._h, part, parts))
 
     def autotune(self, bucket: Bucket, mode: int) -> tuple[Tuning, float]:
         t = Tuning()

@@ -301,6 +301,34 @@ def test_streams_on_a_share_of_the_compute_units():
             assert e.value.code == pgx.PGX_E_ARG
 
 
+def test_runner_confined_to_a_share_of_the_compute_units():
+    """pgx_runner_confine: batches of a runner confined to half the compute
+    units (both batch slots, two batches in flight) give the oracle's verdicts
+    and residues; it is refused while a batch is in flight and for parts out of
+    range; parts = 1 lifts it."""
+    L = gen(337, 2310, 16, 4, 2048, seed_off=5)
+    a, b = L[:4096], L[4096:8192]
+    va, ra = O.verdicts(a, O.MODE_FERMAT, residues=True)
+    vb, rb = O.verdicts(b, O.MODE_FERMAT, residues=True)
+    with pgx.Runner(4096, limbs=16) as run:
+        for part, parts in ((1, 2), (0, 1)):
+            say(f"runner on part {part} of {parts} of the compute units, two batches in flight")
+            run.confine(part, parts)
+            run.submit(a, pgx.MODE_FERMAT, residues=True)
+            run.submit(b, pgx.MODE_FERMAT, residues=True)
+            with pytest.raises(pgx.PgxError) as e:
+                run.confine(0, 2)
+            assert e.value.code == pgx.PGX_E_STATE
+            got_a, got_ra = run.wait()
+            got_b, got_rb = run.wait()
+            assert np.array_equal(got_a, va) and np.array_equal(got_ra, ra)
+            assert np.array_equal(got_b, vb) and np.array_equal(got_rb, rb)
+        for part, parts in ((0, 0), (2, 2), (0, 1 << 20)):
+            with pytest.raises(pgx.PgxError) as e:
+                run.confine(part, parts)
+            assert e.value.code == pgx.PGX_E_ARG
+
+
 def test_base_range_is_checked():
     with pytest.raises(pgx.PgxError) as e:
         pgx.Runner(16, limbs=16, bases=[2, 256])
