@@ -387,7 +387,7 @@ def bench_host_boundary(cands: np.ndarray, reps: int = 3) -> dict:
 
 def bench_dropin(n_x: int = 8, passes: int = 3, keep_dir: str | None = None) -> dict:
     """The drop-in gpu_testing.cpp run by tests/dropin/harness under the
-    reference's own dispatch protocol: GPU_BATCHES = 3 host threads, each with
+    reference's own dispatch protocol: GPU_BATCHES = 2 host threads, each with
     a runner and a GPUBatch of up to 32768 multipliers (candidates built on the
     device), fill_batch / process_finished_batch on the host, the X barrier
     between steps.  Interval: P=337, d=2310, m in [1e9, 1e9+1e7), composites
@@ -451,7 +451,8 @@ def bench_dropin(n_x: int = 8, passes: int = 3, keep_dir: str | None = None) -> 
                  and np.array_equal(found, want_found[:w]) and not want_found[w:].any())
     timed = secs[1:] if len(secs) > 1 else secs
     return {
-        "workload": f"drop-in gpu_testing.cpp under the reference's dispatch (3 GPU threads, "
+        "workload": f"drop-in gpu_testing.cpp under the reference's dispatch (2 GPU threads "
+                    f"on CU halves, "
                     f"batches of 32768 built on the device): P={P} d={d} m in [1e9, 1e9+1e7), "
                     f"primes < 1e8, first {n_x} coprime X",
         "tests_per_pass": tests, "found_per_pass": int(pairs[:, 1].sum()),
@@ -1318,7 +1319,7 @@ def main() -> int:
         extra["dropin"] = bench_dropin(keep_dir=args.dropin_dir)
         dr = extra["dropin"]
         log(f"drop-in: {dr.get('tests_per_s', 0) / 1e6:.2f} M tests/s through gpu_testing.cpp "
-            f"on 3 GPU threads (equal to the device search: {dr.get('equal_to_device_search')})"
+            f"on 2 GPU threads (equal to the device search: {dr.get('equal_to_device_search')})"
             if "tests_per_s" in dr else f"drop-in: {dr}")
     if "rounds" in todo:
         extra["rounds"] = bench_rounds(cands, stream)

@@ -8,8 +8,13 @@
 // What changes for MI355X:
 //  * GPU_BATCH_SIZE is 32768 (reference 8192, gpu_testing.cu:57-58), the
 //    largest size prime_gap_test accepts (gap_search_gpu.cpp:1076-1077): one
-//    lane owns one candidate, so the GPU_BATCHES = 3 batches in flight put
-//    1.5 wavefronts on each of the 1024 SIMDs of 256 CUs.
+//    lane owns one candidate, so one batch is 512 wavefronts.
+//  * GPU_BATCHES is 2 (reference 3), each GPU thread's runner confined to its
+//    own half of the CUs (pgx_runner_confine): each batch then has one SIMD
+//    per wavefront to itself.  Three unconfined threads (1.5 wavefronts per
+//    SIMD, where the older wavefronts keep the issue slots and the batches end
+//    one after another) ran the reference's dispatch 25% slower (36.6-41.9
+//    against 28.5-30.0 ms per pass of 1.9 M tests, same results).
 //  * fill_batch only lists the multipliers m = m_start + m_i; the candidates
 //    N = m * K + x are built on the device (pgx_submit_mk), replacing
 //    mpz_mul_ui + mpz_add_ui per candidate (gpu_testing.cu:171-172) and the
@@ -56,21 +61,20 @@ static const uint32_t kLimbs = kBits / 32;
 #ifdef PGX_GPU_BATCHES
 const size_t GPU_BATCHES = PGX_GPU_BATCHES;
 #else
-const size_t GPU_BATCHES = 3;
+const size_t GPU_BATCHES = 2;
 #endif
 #ifdef PGX_GPU_BATCH_SIZE
 const size_t GPU_BATCH_SIZE = PGX_GPU_BATCH_SIZE;
 #else
 const size_t GPU_BATCH_SIZE = 32 * 1024;
 #endif
-// -DPGX_GPU_CONFINE=1 puts each GPU thread's runner on its own share of the
-// CUs (pgx_runner_confine).  Off by default: under this dispatch it was faster
-// only with 2 threads and 4 hardware queues per process (28.0 against 37.7 ms
-// per pass), slower with 3, 4 or 6 threads or with 16 queues.
+// -DPGX_GPU_CONFINE=0 lets every GPU thread's runner use every CU.  (Confined
+// 3, 4 or 6 threads measured slower than 2: a thread's CUs idle through its
+// host turnaround between batches.)
 #ifdef PGX_GPU_CONFINE
 static const bool kConfine = PGX_GPU_CONFINE;
 #else
-static const bool kConfine = false;
+static const bool kConfine = true;
 #endif
 static const uint32_t kRounds = 1;  // base-2 strong test, as the reference's ROUNDS
 

@@ -1,6 +1,6 @@
 """The drop-in C++ surface: include/miller_rabin.h (test_runner_t over the C
 ABI) and the gpu_testing.cu replacement, run by tests/dropin/harness under the
-reference's own dispatch protocol (three GPU threads, GPUBatch state machine)."""
+reference's own dispatch protocol (GPU_BATCHES GPU threads, GPUBatch state machine)."""
 import os
 import struct
 import subprocess
@@ -146,9 +146,9 @@ def run_harness_case(tmp_path, P, d, m_start, m_inc, max_prime, xs):
 
 
 @pytest.mark.gpu
-def test_dropin_readme_interval_three_gpu_threads(tmp_path):
+def test_dropin_readme_interval(tmp_path):
     """README's interval (P=337, d=2310, m_start=1e7, m_inc=2e5, primes to 1e6),
-    6 X steps: one batch per X, the other two GPU threads retire empty."""
+    6 X steps: one batch per X, the other GPU thread retires empty."""
     assert os.path.exists(HARNESS), "tests/dropin/harness is not built"
     nt, nf = run_harness_case(tmp_path, 337, 2310, 10**7, 200_000, 10**6, [2, 4, 6, 8, 10, 12])
     assert nt > 30_000 and nf > 2_000
